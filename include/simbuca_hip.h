@@ -1,0 +1,192 @@
+/*
+ * simbuca_hip.h -- C ABI of libsimbuca_hip.so, the MI355X (gfx950) force and
+ * Dormand-Prince 5(4) engine for ion-cloud simulation.
+ *
+ * Two export sets:
+ *
+ *  1. cunbody1_force{,_begin,_end}: the exact prototypes the reference binds
+ *     for its GPU Coulomb backend (bin/cunbody.h:11-33, called from
+ *     bin/cunbody.cpp:66,88 and bin/force.cpp:50,86).  Linking this library
+ *     in place of libcunbody1.a makes the reference's force() run its
+ *     direct sum on the MI355X.
+ *
+ *  2. sbk_*: a device-resident, step-level engine.  It replaces the
+ *     reference's Dormand_Prince_5 + error_succes + step() commit
+ *     (bin/ode.cpp:61-218,426-494) and the force() it calls
+ *     (bin/force.cpp:34-175), keeping the whole ion state in HBM across
+ *     stages, steps and retries.  Many independent clouds of different
+ *     sizes share one engine; each has its own time, step size and
+ *     controller.
+ *
+ * All sbk_* functions return 0 on success and a negative SBK_E* code on
+ * failure; sbk_last_error() gives the message.  Host arrays are owned by
+ * the caller and only read/written during the call.  An engine is used by
+ * one host thread at a time and runs on the HIP device that was current
+ * when it was created.
+ */
+#ifndef SIMBUCA_HIP_H
+#define SIMBUCA_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * Export set 1 -- reference CUNBODY interface (bin/cunbody.h:11-33).
+ *
+ * ai[i] = sum_{j<nj} mj[j] (xj[j] - xi[i]) / (|xj[j] - xi[i]|^2 + eps2)^{3/2}
+ * for i < ni.  The reference passes mj = 1, eps2 = (double)1e-23f and
+ * ni = nj = N (bin/cunbody.cpp:31-35,66) and scales the result by
+ * -coulomb_scale e^2 ke / m_i itself (bin/force.cpp:87-93).
+ * _begin enqueues the work and returns; _end (same arguments) blocks until
+ * ai is written.  cunbody1_force does both.  Errors abort the process with
+ * a message, as the reference backends do (bin/nbody_kernel.cu:37-47);
+ * the message is also kept for sbk_last_error().
+ * ---------------------------------------------------------------------- */
+void cunbody1_force(double xj[][3], double mj[], double xi[][3], double eps2, double ai[][3],
+                    int ni, int nj);
+void cunbody1_force_begin(double xj[][3], double mj[], double xi[][3], double eps2,
+                          double ai[][3], int ni, int nj);
+void cunbody1_force_end(double xj[][3], double mj[], double xi[][3], double eps2,
+                        double ai[][3], int ni, int nj);
+
+/* Arithmetic used by cunbody1_force* (no counterpart in the reference,
+ * whose backend precision is fixed inside a closed binary): SBK_FP64
+ * (default) or SBK_MIXED.  Returns the previous setting. */
+int sbk_cunbody_set_precision(int precision);
+
+/* ------------------------------------------------------------------------
+ * Export set 2 -- device-resident engine.
+ * ---------------------------------------------------------------------- */
+
+enum {
+    SBK_OK = 0,
+    SBK_EINVAL = -1,     /* bad argument */
+    SBK_EHIP = -2,       /* HIP runtime error */
+    SBK_ESTATE = -3,     /* call not valid in the engine's current state */
+    SBK_EUNDERFLOW = -4, /* "stepsize underflow created" (bin/ode.cpp:482-483) */
+    SBK_ENOMEM = -5
+};
+
+/* Coulomb arithmetic. */
+enum {
+    SBK_FP64_EXACT = 0,  /* FP64, per-target serial order of bin/force.cpp:158-170:
+                            bit-identical to the reference CPU sum */
+    SBK_FP64 = 1,        /* FP64, tiled and split over sources (fast FP64 baseline) */
+    SBK_MIXED = 2,       /* FP64 separations rounded to FP32, FP32 rsqrt and tile sums,
+                            FP64 accumulation across tiles */
+    SBK_MIXED_KAHAN = 3  /* as SBK_MIXED with compensated FP32 accumulation across tiles */
+};
+
+/* Time at which force() is evaluated inside a step. */
+enum {
+    SBK_TIME_FROZEN = 0, /* every stage sees t_n, as bin/force.cpp:36 does */
+    SBK_TIME_STAGE = 1   /* stage s sees t_n + c_s h */
+};
+
+/* Field model of one cloud.  The first block is the reference's
+ * (bin/trapparameters.h:6-27, bin/ode.h:42-56, bin/force.cpp:70-80); the
+ * rest are Penning-trap terms that this fork of the reference lacks. */
+typedef struct sbk_fields {
+    int    trap;                /* RF+DC quadrupole of bin/force.cpp:70-75 */
+    double kappa, Vdc, Vrf, r0, z0, freq_rf;
+    int    tof;                 /* constant E_kick along x, bin/force.cpp:77-80 */
+    double E_kick;
+    int    coulomb;             /* direct-sum Coulomb, bin/force.cpp:48,153 */
+    double coulomb_scale;
+    double V0, d0;              /* ideal Penning quadrupole V0/(2 d0^2)(z^2-(x^2+y^2)/2); V0=0: off */
+    double Bz;                  /* uniform magnetic field along z [T] */
+    double Ed, fd, phid;        /* dipole excitation along x: Ed cos(2 pi fd t + phid) [V/m] */
+    double Uq, rq, fq, phiq;    /* quadrupole excitation (Uq/rq^2) cos(2 pi fq t + phiq)(x,-y,0) */
+    double delta;               /* buffer-gas friction a -= delta v [1/s] */
+    int    ramp;                /* linear Vrf/Vdc ramp of bin/ion_fly.cpp:235-239 */
+    double ramp_t0, ramp_t1, newVrf, newVdc;
+} sbk_fields;
+
+typedef struct sbk_config {
+    int    precision;   /* SBK_FP64_EXACT .. SBK_MIXED_KAHAN */
+    int    time_mode;   /* SBK_TIME_FROZEN or SBK_TIME_STAGE */
+    int    adaptive;    /* 1: error_succes controller; 0: fixed step (bin/ode.cpp:478-486) */
+    int    use_graph;   /* 1: replay one captured hipGraph per attempt */
+    int    timing;      /* 1: time every Coulomb launch with HIP events */
+    double rtol, atol;  /* the reference hard-codes 1e-7/1e-7 (bin/ode.cpp:515-516) */
+    double h0;          /* initial step (the [ode] timestep of the reference) */
+} sbk_config;
+
+typedef struct sbk_cloud_status {
+    double lifetime;    /* IonCloud::lifetime */
+    double h, hnext, errold, last_err, t_end;
+    int    reject, active, accepted_last, error;
+    long long steps, attempts;
+    int    n;
+    int    slices;      /* source slices of the Coulomb work split */
+} sbk_cloud_status;
+
+typedef struct sbk_stats {
+    double    coulomb_ms;        /* summed HIP-event time of Coulomb launches (timing=1) */
+    long long coulomb_launches;  /* launches counted in coulomb_ms */
+    double    pair_interactions; /* sum over clouds of attempts*7*n(n-1) */
+    long long attempts;          /* sum over clouds */
+    long long steps;             /* accepted steps, summed over clouds */
+    double    ion_steps;         /* sum over clouds of steps*n */
+} sbk_stats;
+
+typedef struct sbk_engine sbk_engine;
+
+void sbk_default_config(sbk_config *cfg);
+void sbk_default_fields(sbk_fields *f);
+int  sbk_create(const sbk_config *cfg, sbk_engine **out);
+int  sbk_destroy(sbk_engine *e);
+
+/* Adds a cloud (n ions, AoS [n][3] positions in m and velocities in m/s,
+ * masses in kg, charges in units of e) at time `lifetime`.  Clouds can be
+ * added until the first call that runs work on the device. */
+int  sbk_add_cloud(sbk_engine *e, int n, const double *pos, const double *vel,
+                   const double *mass, const int *charge, const sbk_fields *f,
+                   double lifetime, int *cloud_id);
+int  sbk_set_fields(sbk_engine *e, int cloud, const sbk_fields *f);
+int  sbk_set_step(sbk_engine *e, int cloud, double h);
+
+/* Runs attempts until every cloud's lifetime reaches lifetime+duration
+ * (the while-loop of move_particles, bin/ion_fly.cpp:83-101), or
+ * max_attempts attempts were launched. */
+int  sbk_advance(sbk_engine *e, double duration, long long max_attempts);
+/* One accepted step for every cloud (step(), bin/ode.cpp:459-494). */
+int  sbk_step(sbk_engine *e);
+/* Enqueues `count` attempts for the active clouds without waiting. */
+int  sbk_attempts(sbk_engine *e, int count);
+int  sbk_synchronize(sbk_engine *e);
+
+int  sbk_get_state(sbk_engine *e, int cloud, double *pos, double *vel);
+int  sbk_get_status(sbk_engine *e, int cloud, sbk_cloud_status *st);
+int  sbk_get_stats(sbk_engine *e, sbk_stats *st);
+int  sbk_reset_stats(sbk_engine *e);
+
+/* force() of bin/force.cpp:34-175 at the cloud's current positions,
+ * velocities and lifetime: derivs[n][3] (an acceleration). */
+int  sbk_eval_force(sbk_engine *e, int cloud, double *derivs);
+/* The Coulomb sum f_j of bin/force.cpp:150-170 alone (derivs_j -= f_j),
+ * computed with `precision`: out[n][3]. */
+int  sbk_eval_coulomb(sbk_engine *e, int cloud, int precision, double *out);
+
+/* Coulomb launch shape.  targets_per_thread in {1,2,4}; slices = number of
+ * source slices per target block (0 = automatic).  Applies to SBK_FP64 and
+ * SBK_MIXED*; SBK_FP64_EXACT always uses one slice. */
+int  sbk_set_launch(sbk_engine *e, int targets_per_thread, int slices);
+/* Times `reps` launches of the Coulomb kernel of the current state's
+ * positions for every cloud; ms_per_launch receives the mean. */
+int  sbk_time_coulomb(sbk_engine *e, int precision, int reps, double *ms_per_launch);
+/* Tries the launch shapes for this engine's clouds and keeps the fastest;
+ * returns the chosen shape.  The table of results is appended to
+ * `table_path` if non-null. */
+int  sbk_autotune(sbk_engine *e, int precision, int *targets_per_thread, int *slices,
+                  const char *table_path);
+
+const char *sbk_last_error(void);
+/* Device name, compute-unit count and clock (MHz) of the current device. */
+int  sbk_device_info(char *name, int name_len, int *cus, int *clock_mhz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,96 @@
+/*
+ * simbuca_oracle.h -- CPU FP64 restatement of the reference's force and
+ * Dormand-Prince 5(4) path.  TEST INFRASTRUCTURE ONLY: this is the checker
+ * that the HIP engine is compared against; it is never linked into, or
+ * called by, the product library.
+ *
+ * Reference files restated (paths relative to the reference tree):
+ *   bin/globals.h:4-15          physical constants
+ *   bin/ion.cpp:3-11            mass_amu * amu in long double
+ *   bin/force.cpp:34-82         trap quadrupole (RF+DC) and TOF terms
+ *   bin/force.cpp:147-175       FP64 direct-sum Coulomb (non-MPI branch)
+ *   bin/ode.cpp:61-218          Dormand_Prince_5 stages and error norm
+ *   bin/ode.cpp:426-457         error_succes step-size controller
+ *   bin/ode.cpp:459-494         step() retry loop and commit
+ *   bin/ion_fly.cpp:83-101      move_particles loop incl. trap ramp
+ *
+ * Extensions that the fork does not contain (Penning B field, ideal
+ * Penning quadrupole, dipole/quadrupole RF excitation, buffer-gas
+ * friction, stage-time evaluation) are defined here from their closed
+ * forms; their tests pin them against analytic solutions.
+ */
+#ifndef SIMBUCA_ORACLE_H
+#define SIMBUCA_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_fields {
+    /* reference trap term, force.cpp:70-75 */
+    int    trap;
+    double kappa, Vdc, Vrf, r0, z0, freq_rf;
+    /* reference TOF kick, force.cpp:77-80 */
+    int    tof;
+    double E_kick;
+    /* reference Coulomb switch and scale, force.cpp:48,153 */
+    int    coulomb;
+    double coulomb_scale;
+    /* ideal Penning quadrupole potential V0/(2 d0^2) (z^2 - (x^2+y^2)/2); off if V0 == 0 */
+    double V0, d0;
+    /* uniform magnetic field along z [T] */
+    double Bz;
+    /* dipole RF excitation along x: Ed cos(2 pi fd t + phid) [V/m] */
+    double Ed, fd, phid;
+    /* quadrupole RF excitation: (Uq/rq^2) cos(2 pi fq t + phiq) (x, -y, 0) [V/m] */
+    double Uq, rq, fq, phiq;
+    /* buffer-gas friction a -= delta v [1/s] */
+    double delta;
+    /* trap ramp of move_particles (ion_fly.cpp:235-239) */
+    int    ramp;
+    double ramp_t0, ramp_t1, newVrf, newVdc;
+} orc_fields;
+
+enum { ORC_TIME_FROZEN = 0, ORC_TIME_STAGE = 1 };
+
+typedef struct orc_cloud orc_cloud;
+
+/* Ion mass in kg exactly as Ion::Ion computes it (long double product). */
+double orc_mass_kg(double mass_amu);
+
+/* Restatement of force(): derivs[n][3] (an acceleration) at time t. */
+void orc_force(int n, const double *pos, const double *vel, const double *mass,
+               const int *charge, const orc_fields *f, double t, double *derivs);
+
+/* Coulomb sum f_j of force.cpp:150-170 for the listed target rows only
+ * (derivs_j -= f_j in the reference); out[nrows][3]. */
+void orc_coulomb_rows(int n, const double *pos, const double *mass, const int *charge,
+                      double coulomb_scale, const int *rows, int nrows, double *out);
+
+/* Generic direct sum of the cunbody1_force interface:
+ * ai[i] = sum_j mj[j] (xj[j]-xi[i]) / (|xj[j]-xi[i]|^2 + eps2)^{3/2}. */
+void orc_cunbody_force(const double *xj, const double *mj, const double *xi, double eps2,
+                       double *ai, int ni, int nj);
+
+/* Controller of ode.cpp:426-457 on a scalar error. Returns 1 on accept. */
+int orc_controller(double err, double *h, double *hnext, double *errold, int *reject);
+
+orc_cloud *orc_create(int n, const double *pos, const double *vel, const double *mass,
+                      const int *charge, const orc_fields *f, double lifetime, double h0,
+                      double rtol, double atol, int adaptive, int time_mode);
+void   orc_destroy(orc_cloud *c);
+void   orc_set_fields(orc_cloud *c, const orc_fields *f);
+/* One Dormand_Prince_5 attempt from (pos, vel); leaves y_{n+1} in pos2/vel2. */
+double orc_dp5_attempt(orc_cloud *c);
+/* step(): retries until accepted, commits. 0 = ok, -1 = step-size underflow. */
+int    orc_step(orc_cloud *c);
+/* move_particles(): steps while lifetime < t_end. Records h before each
+ * step into h_hist (if non-null, up to cap). Returns steps taken or -1. */
+long   orc_advance(orc_cloud *c, double t_end, long max_steps, double *h_hist, long cap);
+void   orc_get_state(const orc_cloud *c, double *pos, double *vel, double *scalars);
+/* scalars: [lifetime, h, hnext, errold, reject, steps, attempts, last_err] */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,331 @@
+"""simbuca_amd -- MI355X force and Dormand-Prince 5(4) engine for ion clouds.
+
+Python binding of ``libsimbuca_hip.so`` (C ABI in ``include/simbuca_hip.h``).
+The compute path is the HIP library; this module only marshals arrays.  If
+the library has not been built it raises instead of falling back to any CPU
+implementation.
+
+Names follow the reference (mgrau/Simbuca): an ``Engine`` holds one or more
+ion clouds (``IonCloud``, bin/ioncloud.h:17-71) and advances them with the
+device-resident equivalent of ``step()``/``move_particles()``
+(bin/ode.cpp:459-494, bin/ion_fly.cpp:17-122); ``Fields`` mirrors the trap
+parameters of bin/trapparameters.h:6-27 plus the Penning-trap terms.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsimbuca_hip.so")
+
+FP64_EXACT, FP64, MIXED, MIXED_KAHAN = 0, 1, 2, 3
+PRECISIONS = {"fp64_exact": FP64_EXACT, "fp64": FP64, "mixed": MIXED, "mixed_kahan": MIXED_KAHAN}
+TIME_FROZEN, TIME_STAGE = 0, 1
+
+SBK_EUNDERFLOW = -4
+
+# bin/globals.h:6-15
+EL_CHARGE = 1.60217733e-19
+AMU = 1.6605402e-27
+KE = 8.987551787e9
+KB = 1.3806504e-23
+
+
+class SimbucaError(RuntimeError):
+    pass
+
+
+class StepsizeUnderflow(SimbucaError):
+    """The reference throws "stepsize underflow created" (bin/ode.cpp:482-483)."""
+
+
+class Fields(ctypes.Structure):
+    """``sbk_fields`` of include/simbuca_hip.h."""
+
+    _fields_ = [
+        ("trap", ctypes.c_int),
+        ("kappa", ctypes.c_double), ("Vdc", ctypes.c_double), ("Vrf", ctypes.c_double),
+        ("r0", ctypes.c_double), ("z0", ctypes.c_double), ("freq_rf", ctypes.c_double),
+        ("tof", ctypes.c_int), ("E_kick", ctypes.c_double),
+        ("coulomb", ctypes.c_int), ("coulomb_scale", ctypes.c_double),
+        ("V0", ctypes.c_double), ("d0", ctypes.c_double),
+        ("Bz", ctypes.c_double),
+        ("Ed", ctypes.c_double), ("fd", ctypes.c_double), ("phid", ctypes.c_double),
+        ("Uq", ctypes.c_double), ("rq", ctypes.c_double), ("fq", ctypes.c_double),
+        ("phiq", ctypes.c_double),
+        ("delta", ctypes.c_double),
+        ("ramp", ctypes.c_int), ("ramp_t0", ctypes.c_double), ("ramp_t1", ctypes.c_double),
+        ("newVrf", ctypes.c_double), ("newVdc", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("precision", ctypes.c_int), ("time_mode", ctypes.c_int), ("adaptive", ctypes.c_int),
+        ("use_graph", ctypes.c_int), ("timing", ctypes.c_int),
+        ("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("h0", ctypes.c_double),
+    ]
+
+
+class CloudStatus(ctypes.Structure):
+    _fields_ = [
+        ("lifetime", ctypes.c_double), ("h", ctypes.c_double), ("hnext", ctypes.c_double),
+        ("errold", ctypes.c_double), ("last_err", ctypes.c_double), ("t_end", ctypes.c_double),
+        ("reject", ctypes.c_int), ("active", ctypes.c_int), ("accepted_last", ctypes.c_int),
+        ("error", ctypes.c_int),
+        ("steps", ctypes.c_longlong), ("attempts", ctypes.c_longlong),
+        ("n", ctypes.c_int), ("slices", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("coulomb_ms", ctypes.c_double), ("coulomb_launches", ctypes.c_longlong),
+        ("pair_interactions", ctypes.c_double), ("attempts", ctypes.c_longlong),
+        ("steps", ctypes.c_longlong), ("ion_steps", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+# every symbol include/simbuca_hip.h declares
+EXPORTS = [
+    "cunbody1_force", "cunbody1_force_begin", "cunbody1_force_end", "sbk_cunbody_set_precision",
+    "sbk_default_config", "sbk_default_fields", "sbk_create", "sbk_destroy", "sbk_add_cloud",
+    "sbk_set_fields", "sbk_set_step", "sbk_advance", "sbk_step", "sbk_attempts",
+    "sbk_synchronize", "sbk_get_state", "sbk_get_status", "sbk_get_stats", "sbk_reset_stats",
+    "sbk_eval_force", "sbk_eval_coulomb", "sbk_set_launch", "sbk_time_coulomb", "sbk_autotune",
+    "sbk_last_error", "sbk_device_info",
+]
+
+
+def lib():
+    """Loads libsimbuca_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SimbucaError(
+            f"{LIB_PATH} is missing: build it with `make -C simbuca_amd/csrc` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    vp = ctypes.c_void_p
+    c_int, c_dbl, c_ll = ctypes.c_int, ctypes.c_double, ctypes.c_longlong
+    sig = {
+        "cunbody1_force": (None, [dp, dp, dp, c_dbl, dp, c_int, c_int]),
+        "cunbody1_force_begin": (None, [dp, dp, dp, c_dbl, dp, c_int, c_int]),
+        "cunbody1_force_end": (None, [dp, dp, dp, c_dbl, dp, c_int, c_int]),
+        "sbk_cunbody_set_precision": (c_int, [c_int]),
+        "sbk_default_config": (None, [ctypes.POINTER(Config)]),
+        "sbk_default_fields": (None, [ctypes.POINTER(Fields)]),
+        "sbk_create": (c_int, [ctypes.POINTER(Config), ctypes.POINTER(vp)]),
+        "sbk_destroy": (c_int, [vp]),
+        "sbk_add_cloud": (c_int, [vp, c_int, dp, dp, dp, ip, ctypes.POINTER(Fields), c_dbl, ip]),
+        "sbk_set_fields": (c_int, [vp, c_int, ctypes.POINTER(Fields)]),
+        "sbk_set_step": (c_int, [vp, c_int, c_dbl]),
+        "sbk_advance": (c_int, [vp, c_dbl, c_ll]),
+        "sbk_step": (c_int, [vp]),
+        "sbk_attempts": (c_int, [vp, c_int]),
+        "sbk_synchronize": (c_int, [vp]),
+        "sbk_get_state": (c_int, [vp, c_int, dp, dp]),
+        "sbk_get_status": (c_int, [vp, c_int, ctypes.POINTER(CloudStatus)]),
+        "sbk_get_stats": (c_int, [vp, ctypes.POINTER(Stats)]),
+        "sbk_reset_stats": (c_int, [vp]),
+        "sbk_eval_force": (c_int, [vp, c_int, dp]),
+        "sbk_eval_coulomb": (c_int, [vp, c_int, c_int, dp]),
+        "sbk_set_launch": (c_int, [vp, c_int, c_int]),
+        "sbk_time_coulomb": (c_int, [vp, c_int, c_int, dp]),
+        "sbk_autotune": (c_int, [vp, c_int, ip, ip, ctypes.c_char_p]),
+        "sbk_last_error": (ctypes.c_char_p, []),
+        "sbk_device_info": (c_int, [ctypes.c_char_p, c_int, ip, ip]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().sbk_last_error().decode(errors="replace")
+        if rc == SBK_EUNDERFLOW:
+            raise StepsizeUnderflow(msg)
+        raise SimbucaError(f"libsimbuca_hip error {rc}: {msg}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def default_fields(**kw) -> Fields:
+    """Reference defaults (bin/trapparameters.cpp:2-10) with keyword overrides."""
+    f = Fields()
+    lib().sbk_default_fields(ctypes.byref(f))
+    for k, v in kw.items():
+        if not hasattr(f, k):
+            raise KeyError(k)
+        setattr(f, k, v)
+    return f
+
+
+def device_info():
+    name = ctypes.create_string_buffer(128)
+    cus = ctypes.c_int(); clk = ctypes.c_int()
+    _check(lib().sbk_device_info(name, 128, ctypes.byref(cus), ctypes.byref(clk)))
+    return {"name": name.value.decode(), "cus": cus.value, "clock_mhz": clk.value}
+
+
+class Engine:
+    """Device-resident clouds advanced by Dormand-Prince 5(4) on the GPU."""
+
+    def __init__(self, precision="mixed", time_mode=TIME_FROZEN, adaptive=True, use_graph=True,
+                 timing=False, rtol=1e-7, atol=1e-7, h0=1e-9):
+        L = lib()
+        cfg = Config()
+        L.sbk_default_config(ctypes.byref(cfg))
+        cfg.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        cfg.time_mode = int(time_mode)
+        cfg.adaptive = 1 if adaptive else 0
+        cfg.use_graph = 1 if use_graph else 0
+        cfg.timing = 1 if timing else 0
+        cfg.rtol, cfg.atol, cfg.h0 = float(rtol), float(atol), float(h0)
+        h = ctypes.c_void_p()
+        _check(L.sbk_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self.config = cfg
+        self.sizes = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sbk_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add_cloud(self, pos, vel, mass, charge, fields: Fields, lifetime=0.0) -> int:
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        mass = np.ascontiguousarray(mass, dtype=np.float64).ravel()
+        charge = np.ascontiguousarray(charge, dtype=np.int32).ravel()
+        n = pos.shape[0]
+        if vel.shape[0] != n or mass.size != n or charge.size != n:
+            raise ValueError("pos, vel, mass and charge must describe the same ions")
+        cid = ctypes.c_int()
+        _check(lib().sbk_add_cloud(self._h, n, _dp(pos), _dp(vel), _dp(mass), _ip(charge),
+                                   ctypes.byref(fields), float(lifetime), ctypes.byref(cid)))
+        self.sizes.append(n)
+        return cid.value
+
+    def set_fields(self, cloud, fields: Fields):
+        _check(lib().sbk_set_fields(self._h, cloud, ctypes.byref(fields)))
+
+    def set_step(self, cloud, h):
+        _check(lib().sbk_set_step(self._h, cloud, float(h)))
+
+    def set_launch(self, targets_per_thread=2, slices=0):
+        _check(lib().sbk_set_launch(self._h, int(targets_per_thread), int(slices)))
+
+    def advance(self, duration, max_attempts=1 << 62):
+        _check(lib().sbk_advance(self._h, float(duration), int(max_attempts)))
+
+    def step(self):
+        _check(lib().sbk_step(self._h))
+
+    def attempts(self, count, open_ended=True):
+        """Enqueues `count` attempts without waiting.  With open_ended, first
+        makes every cloud active with no end time (sbk_advance(inf, 0))."""
+        if open_ended:
+            _check(lib().sbk_advance(self._h, float("inf"), 0))
+        _check(lib().sbk_attempts(self._h, int(count)))
+
+    def synchronize(self):
+        _check(lib().sbk_synchronize(self._h))
+
+    def state(self, cloud=0):
+        n = self.sizes[cloud]
+        pos = np.zeros((n, 3)); vel = np.zeros((n, 3))
+        _check(lib().sbk_get_state(self._h, cloud, _dp(pos), _dp(vel)))
+        return pos, vel
+
+    def status(self, cloud=0):
+        st = CloudStatus()
+        _check(lib().sbk_get_status(self._h, cloud, ctypes.byref(st)))
+        return st.as_dict()
+
+    def stats(self):
+        st = Stats()
+        _check(lib().sbk_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def reset_stats(self):
+        _check(lib().sbk_reset_stats(self._h))
+
+    def eval_force(self, cloud=0):
+        out = np.zeros((self.sizes[cloud], 3))
+        _check(lib().sbk_eval_force(self._h, cloud, _dp(out)))
+        return out
+
+    def eval_coulomb(self, cloud=0, precision="fp64_exact"):
+        p = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        out = np.zeros((self.sizes[cloud], 3))
+        _check(lib().sbk_eval_coulomb(self._h, cloud, p, _dp(out)))
+        return out
+
+    def time_coulomb(self, precision="mixed", reps=5):
+        p = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        ms = ctypes.c_double()
+        _check(lib().sbk_time_coulomb(self._h, p, int(reps), ctypes.byref(ms)))
+        return ms.value
+
+    def autotune(self, precision="mixed", table_path=None):
+        p = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        t = ctypes.c_int(); s = ctypes.c_int()
+        _check(lib().sbk_autotune(self._h, p, ctypes.byref(t), ctypes.byref(s),
+                                  table_path.encode() if table_path else None))
+        return t.value, s.value
+
+
+def cunbody1_force(xj, mj, xi, eps2, precision=None):
+    """The reference's cunbody1_force (bin/cunbody.h:11-17) through the C ABI."""
+    L = lib()
+    xj = np.ascontiguousarray(xj, dtype=np.float64).reshape(-1, 3)
+    xi = np.ascontiguousarray(xi, dtype=np.float64).reshape(-1, 3)
+    mj = np.ascontiguousarray(mj, dtype=np.float64).ravel()
+    ai = np.zeros_like(xi)
+    old = None
+    if precision is not None:
+        p = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        old = L.sbk_cunbody_set_precision(p)
+    try:
+        L.cunbody1_force(_dp(xj), _dp(mj), _dp(xi), float(eps2), _dp(ai), xi.shape[0], xj.shape[0])
+    finally:
+        if old is not None:
+            L.sbk_cunbody_set_precision(old)
+    return ai

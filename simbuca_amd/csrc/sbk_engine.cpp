@@ -1,0 +1,953 @@
+// sbk_engine.cpp -- host side of the device-resident Dormand-Prince engine
+// (export set sbk_* of include/simbuca_hip.h).
+//
+// The engine replaces the reference's per-step host loop
+// (bin/ion_fly.cpp:83-101 -> step(), bin/ode.cpp:459-494 ->
+// Dormand_Prince_5, bin/ode.cpp:61-218 -> force(), bin/force.cpp:34-175).
+// Ion state lives in HBM from sbk_add_cloud to sbk_destroy; one attempt is
+// 7 x (Coulomb kernel + stage kernel) + controller + commit, all enqueued
+// on the engine's stream (optionally replayed from one hipGraph), and the
+// host only reads the per-cloud scalars back between batches of attempts.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/simbuca_hip.h"
+#include "sbk_internal.h"
+
+namespace {
+
+// bin/globals.h:6-10
+const double kPi = 3.14159265358979323;
+const double kElCharge = 1.60217733e-19;
+const double kKe = 8.987551787E9;
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(SBK_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),   \
+                        __FILE__, __LINE__);                                               \
+    } while (0)
+
+struct HostCloud {
+    int n = 0, off = 0;
+    sbk_fields f{};
+    double lifetime = 0.0;
+    std::vector<double> pos, vel, mass;
+    std::vector<int> charge;
+};
+
+struct ItemList {
+    sbk::CoulombItem *d_items = nullptr;
+    int *d_slices = nullptr;
+    int nitems = 0;
+    int max_slices = 1;
+    std::vector<int> slices;
+};
+
+}  // namespace
+
+struct sbk_engine {
+    sbk_config cfg{};
+    int device = 0;
+    int cus = 256;
+    hipStream_t stream = nullptr;
+    bool finalized = false;
+    std::vector<HostCloud> clouds;
+    int ntot = 0;
+    long long stride = 0;
+
+    // device memory
+    double *d_f64 = nullptr;   // all FP64 SoA arrays
+    float *d_qf = nullptr;
+    int *d_cloud_of = nullptr;
+    double *d_cp = nullptr;
+    int cp_slices = 0;
+    sbk::CloudParams *d_params = nullptr;
+    sbk::CloudState *d_state = nullptr;
+    sbk::CloudState *h_state = nullptr;   // pinned mirror
+    double *d_kc = nullptr, *d_qd = nullptr;
+    sbk::EngineDev dev{};
+
+    // launch shape
+    int tpt = 2;
+    int slices_req = 0;
+    std::map<std::tuple<int, int, int, int>, ItemList> lists;
+
+    // graph of one attempt
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+
+    // timing
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t ev_used = 0;
+    double coulomb_ms = 0.0;
+    long long coulomb_launches = 0;
+    std::vector<long long> attempts_at_reset, steps_at_reset;
+};
+
+namespace {
+
+sbk::CloudParams make_params(const sbk_fields &f, int off, int n)
+{
+    sbk::CloudParams p{};
+    p.trap = f.trap;
+    p.tof = f.tof;
+    p.coulomb = f.coulomb;
+    p.ramp = f.ramp;
+    p.kappa = f.kappa;
+    p.Vdc = f.Vdc;
+    p.Vrf = f.Vrf;
+    p.r02 = f.r0 * f.r0;                       // bin/force.cpp:42-43
+    p.z02 = f.z0 * f.z0;
+    p.omega_rf = f.freq_rf * 2 * kPi;          // bin/force.cpp:44
+    p.E_kick = f.E_kick;
+    p.ramp_t0 = f.ramp_t0;
+    p.ramp_t1 = f.ramp_t1;
+    p.newVrf = f.newVrf;
+    p.newVdc = f.newVdc;
+    p.cpen = (f.V0 != 0.0) ? f.V0 / (2.0 * f.d0 * f.d0) : 0.0;
+    p.Bz = f.Bz;
+    p.Ed = f.Ed;
+    p.fd = f.fd;
+    p.phid = f.phid;
+    p.Uq_rq2 = (f.Uq != 0.0) ? f.Uq / (f.rq * f.rq) : 0.0;
+    p.fq = f.fq;
+    p.phiq = f.phiq;
+    p.delta = f.delta;
+    p.off = off;
+    p.n = n;
+    p.slices = 1;
+    return p;
+}
+
+int kp(int precision)
+{
+    return precision;  // SBK_* and sbk::KP_* share values
+}
+
+void destroy_graph(sbk_engine *e)
+{
+    if (e->graph_exec) hipGraphExecDestroy(e->graph_exec);
+    if (e->graph) hipGraphDestroy(e->graph);
+    e->graph_exec = nullptr;
+    e->graph = nullptr;
+}
+
+void free_lists(sbk_engine *e)
+{
+    for (auto &kv : e->lists) {
+        hipFree(kv.second.d_items);
+        hipFree(kv.second.d_slices);
+    }
+    e->lists.clear();
+    destroy_graph(e);
+}
+
+// Builds (or returns the cached) Coulomb work list for a precision and
+// launch shape.  `only_coulomb` keeps the clouds whose Coulomb switch is on.
+int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
+{
+    const int tpt = (precision == SBK_FP64_EXACT) ? 1 : e->tpt;
+    const int sreq = (precision == SBK_FP64_EXACT) ? 1 : e->slices_req;
+    auto key = std::make_tuple(precision, tpt, sreq, only_coulomb ? 1 : 0);
+    auto it = e->lists.find(key);
+    if (it != e->lists.end()) {
+        *out = &it->second;
+        return 0;
+    }
+    const int blk = sbk::kBlock * tpt;
+    long long total_blocks = 0;
+    for (auto &c : e->clouds)
+        if (c.n > 0 && (!only_coulomb || c.f.coulomb)) total_blocks += (c.n + blk - 1) / blk;
+    const long long target = 8LL * e->cus;
+    ItemList L;
+    std::vector<sbk::CoulombItem> items;
+    L.slices.assign(e->clouds.size(), 1);
+    for (size_t ci = 0; ci < e->clouds.size(); ++ci) {
+        const HostCloud &c = e->clouds[ci];
+        if (c.n <= 0 || (only_coulomb && !c.f.coulomb)) continue;
+        const int ib = (c.n + blk - 1) / blk;
+        int S = 1;
+        if (precision != SBK_FP64_EXACT) {
+            if (sreq > 0)
+                S = sreq;
+            else {
+                long long want = total_blocks > 0 ? (target + total_blocks - 1) / total_blocks : 1;
+                long long cap = std::max(1, c.n / 1024);
+                S = (int)std::max(1LL, std::min(want, cap));
+            }
+        }
+        S = std::max(1, std::min(S, c.n));
+        // slice length: a multiple of the FP32 sub-tile
+        int chunk = (c.n + S - 1) / S;
+        chunk = ((chunk + sbk::kSubTile - 1) / sbk::kSubTile) * sbk::kSubTile;
+        S = (c.n + chunk - 1) / chunk;
+        L.slices[ci] = S;
+        L.max_slices = std::max(L.max_slices, S);
+        for (int s = 0; s < S; ++s) {
+            const int j0 = c.off + s * chunk, j1 = c.off + std::min(c.n, (s + 1) * chunk);
+            for (int b = 0; b < ib; ++b) {
+                sbk::CoulombItem w{};
+                w.cloud = (int)ci;
+                w.i0 = c.off + b * blk;
+                w.i1 = c.off + std::min(c.n, (b + 1) * blk);
+                w.j0 = j0;
+                w.j1 = j1;
+                w.slice = s;
+                items.push_back(w);
+            }
+        }
+    }
+    L.nitems = (int)items.size();
+    if (L.nitems > 0) {
+        HIP_TRY(hipMalloc(&L.d_items, sizeof(sbk::CoulombItem) * items.size()));
+        HIP_TRY(hipMemcpy(L.d_items, items.data(), sizeof(sbk::CoulombItem) * items.size(),
+                          hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipMalloc(&L.d_slices, sizeof(int) * std::max<size_t>(1, L.slices.size())));
+    HIP_TRY(hipMemcpy(L.d_slices, L.slices.data(), sizeof(int) * L.slices.size(),
+                      hipMemcpyHostToDevice));
+    if (L.max_slices > e->cp_slices) {
+        destroy_graph(e);
+        if (e->d_cp) HIP_TRY(hipFree(e->d_cp));
+        e->d_cp = nullptr;
+        HIP_TRY(hipMalloc(&e->d_cp, sizeof(double) * 3 * e->stride * L.max_slices));
+        e->cp_slices = L.max_slices;
+        e->dev.CP = e->d_cp;
+    }
+    auto res = e->lists.emplace(key, L);
+    *out = &res.first->second;
+    return 0;
+}
+
+int upload_params(sbk_engine *e)
+{
+    std::vector<sbk::CloudParams> ps;
+    for (auto &c : e->clouds) ps.push_back(make_params(c.f, c.off, c.n));
+    HIP_TRY(hipMemcpyAsync(e->d_params, ps.data(), sizeof(sbk::CloudParams) * ps.size(),
+                           hipMemcpyHostToDevice, e->stream));
+    return 0;
+}
+
+// per-ion constants of one cloud: qm = charge*e/mass (bin/force.cpp:71),
+// kc = coulomb_scale*ke*e*charge*e/mass (bin/force.cpp:153)
+void ion_constants(const HostCloud &c, double *qm, double *kc, double *qd, float *qf)
+{
+    for (int i = 0; i < c.n; ++i) {
+        qm[i] = c.charge[i] * kElCharge / c.mass[i];
+        kc[i] = c.f.coulomb_scale * kKe * kElCharge * c.charge[i] * kElCharge / c.mass[i];
+        qd[i] = (double)c.charge[i];
+        qf[i] = (float)c.charge[i];
+    }
+}
+
+// index of the FP64 SoA arrays inside d_f64
+enum {
+    A_P0 = 0, A_V0 = 3, A_PS = 6, A_KV = 9 /* KV2..KV7: 18 */, A_KA = 27 /* KA1..KA7: 21 */,
+    A_ERR = 48, A_DER = 49 /* 3 */, A_QM = 52, A_KC = 53, A_QD = 54, A_COUNT = 55
+};
+
+int finalize(sbk_engine *e)
+{
+    if (e->finalized) return 0;
+    if (e->clouds.empty()) return fail(SBK_ESTATE, "engine has no clouds");
+    e->ntot = 0;
+    for (auto &c : e->clouds) {
+        c.off = e->ntot;
+        e->ntot += c.n;
+    }
+    e->stride = std::max<long long>(64, ((long long)e->ntot + 63) / 64 * 64);
+    const long long S = e->stride;
+    HIP_TRY(hipMalloc(&e->d_f64, sizeof(double) * S * A_COUNT));
+    HIP_TRY(hipMemsetAsync(e->d_f64, 0, sizeof(double) * S * A_COUNT, e->stream));
+    HIP_TRY(hipMalloc(&e->d_qf, sizeof(float) * S));
+    HIP_TRY(hipMalloc(&e->d_cloud_of, sizeof(int) * S));
+    HIP_TRY(hipMalloc(&e->d_params, sizeof(sbk::CloudParams) * e->clouds.size()));
+    HIP_TRY(hipMalloc(&e->d_state, sizeof(sbk::CloudState) * e->clouds.size()));
+    HIP_TRY(hipHostMalloc(&e->h_state, sizeof(sbk::CloudState) * e->clouds.size(),
+                          hipHostMallocDefault));
+
+    // host staging in SoA, then one copy per array
+    std::vector<double> soa(S * 6, 0.0), cst(S * 3, 0.0);
+    std::vector<float> qf(S, 0.f);
+    std::vector<int> cof(S, 0);
+    for (size_t ci = 0; ci < e->clouds.size(); ++ci) {
+        HostCloud &c = e->clouds[ci];
+        for (int i = 0; i < c.n; ++i) {
+            for (int k = 0; k < 3; ++k) {
+                soa[k * S + c.off + i] = c.pos[3 * i + k];
+                soa[(3 + k) * S + c.off + i] = c.vel[3 * i + k];
+            }
+            cof[c.off + i] = (int)ci;
+        }
+        ion_constants(c, &cst[c.off], &cst[S + c.off], &cst[2 * S + c.off], &qf[c.off]);
+    }
+    double *base = e->d_f64;
+    HIP_TRY(hipMemcpyAsync(base + A_P0 * S, soa.data(), sizeof(double) * S * 6,
+                           hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(base + A_QM * S, cst.data(), sizeof(double) * S * 3,
+                           hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->d_qf, qf.data(), sizeof(float) * S, hipMemcpyHostToDevice,
+                           e->stream));
+    HIP_TRY(hipMemcpyAsync(e->d_cloud_of, cof.data(), sizeof(int) * S, hipMemcpyHostToDevice,
+                           e->stream));
+    std::vector<sbk::CloudState> st(e->clouds.size());
+    for (size_t ci = 0; ci < e->clouds.size(); ++ci) {
+        sbk::CloudState s{};
+        s.t = e->clouds[ci].lifetime;
+        s.h = e->cfg.h0;
+        s.hnext = e->cfg.h0;       // _ode_vars::Init_ode, bin/ode.cpp:539-546
+        s.errold = 1.0e-4;
+        s.t_end = s.t;
+        s.budget = -1;
+        st[ci] = s;
+    }
+    HIP_TRY(hipMemcpyAsync(e->d_state, st.data(), sizeof(sbk::CloudState) * st.size(),
+                           hipMemcpyHostToDevice, e->stream));
+    int rc = upload_params(e);
+    if (rc) return rc;
+
+    sbk::EngineDev &d = e->dev;
+    for (int k = 0; k < 3; ++k) {
+        d.P0[k] = base + (A_P0 + k) * S;
+        d.V0[k] = base + (A_V0 + k) * S;
+        d.PS[k] = base + (A_PS + k) * S;
+        d.KV[0][k] = nullptr;
+        d.KA[0][k] = nullptr;
+        d.KV[1][k] = d.V0[k];
+        for (int s = 2; s <= 7; ++s) d.KV[s][k] = base + (A_KV + (s - 2) * 3 + k) * S;
+        for (int s = 1; s <= 7; ++s) d.KA[s][k] = base + (A_KA + (s - 1) * 3 + k) * S;
+    }
+    d.ERR = base + A_ERR * S;
+    d.derivs_out = base + A_DER * S;
+    d.stride = S;
+    d.qm = base + A_QM * S;
+    e->d_kc = base + A_KC * S;
+    e->d_qd = base + A_QD * S;
+    d.cloud_of = e->d_cloud_of;
+    d.params = e->d_params;
+    d.state = e->d_state;
+    d.ntot = e->ntot;
+    d.nclouds = (int)e->clouds.size();
+    d.time_mode = e->cfg.time_mode;
+    d.adaptive = e->cfg.adaptive;
+    d.rtol = e->cfg.rtol;
+    d.atol = e->cfg.atol;
+    d.CP = nullptr;
+    d.slices = nullptr;
+    e->attempts_at_reset.assign(e->clouds.size(), 0);
+    e->steps_at_reset.assign(e->clouds.size(), 0);
+    e->finalized = true;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+sbk::CoulombArgs coulomb_args(sbk_engine *e, const ItemList &L, double *const *pos, bool gated)
+{
+    sbk::CoulombArgs a{};
+    a.tx = pos[0];
+    a.ty = pos[1];
+    a.tz = pos[2];
+    a.sx = pos[0];
+    a.sy = pos[1];
+    a.sz = pos[2];
+    a.sqd = e->d_qd;
+    a.sqf = e->d_qf;
+    a.kc = e->d_kc;
+    a.out = e->d_cp;
+    a.stride = e->stride;
+    a.items = L.d_items;
+    a.nitems = L.nitems;
+    a.self_skip = 1;
+    a.state = gated ? e->d_state : nullptr;
+    a.eps_f = 1e-23f;   // (float) softening of bin/cunbody.cpp:11; keeps the self pair finite
+    a.eps_d = 0.0;      // FP64-exact: no softening (bin/force.cpp:163)
+    return a;
+}
+
+int launch_coulomb_timed(sbk_engine *e, int precision, const sbk::CoulombArgs &a)
+{
+    std::pair<hipEvent_t, hipEvent_t> *evp = nullptr;
+    if (e->cfg.timing) {
+        if (e->ev_used == e->ev.size()) {
+            hipEvent_t a0, a1;
+            HIP_TRY(hipEventCreate(&a0));
+            HIP_TRY(hipEventCreate(&a1));
+            e->ev.emplace_back(a0, a1);
+        }
+        evp = &e->ev[e->ev_used++];
+        HIP_TRY(hipEventRecord(evp->first, e->stream));
+    }
+    sbk::CoulombArgs b = a;
+    if (precision == SBK_FP64) b.eps_d = 1e-40;   // the guard of bin/force.cpp:136
+    if (sbk::launch_coulomb(kp(precision), e->tpt, b, e->stream))
+        return fail(SBK_EHIP, "Coulomb kernel launch failed: %s",
+                    hipGetErrorString(hipGetLastError()));
+    if (evp) HIP_TRY(hipEventRecord(evp->second, e->stream));
+    return 0;
+}
+
+int harvest_events(sbk_engine *e)
+{
+    for (size_t i = 0; i < e->ev_used; ++i) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e->ev[i].first, e->ev[i].second));
+        e->coulomb_ms += ms;
+        e->coulomb_launches++;
+    }
+    e->ev_used = 0;
+    return 0;
+}
+
+// Enqueues one attempt for all active clouds.
+int enqueue_attempt(sbk_engine *e)
+{
+    ItemList *L = nullptr;
+    bool any_coulomb = false;
+    for (auto &c : e->clouds) any_coulomb |= (c.f.coulomb && c.n > 0);
+    int rc = get_list(e, e->cfg.precision, true, &L);
+    if (rc) return rc;
+    sbk::EngineDev d = e->dev;
+    d.slices = L->d_slices;
+    for (int s = 1; s <= 7; ++s) {
+        if (any_coulomb && L->nitems > 0) {
+            sbk::CoulombArgs a = coulomb_args(e, *L, s == 1 ? d.P0 : d.PS, true);
+            rc = launch_coulomb_timed(e, e->cfg.precision, a);
+            if (rc) return rc;
+        }
+        if (sbk::launch_stage(s, d, e->stream)) return fail(SBK_EHIP, "stage kernel launch failed");
+    }
+    if (sbk::launch_controller(d, e->stream)) return fail(SBK_EHIP, "controller launch failed");
+    if (sbk::launch_commit(d, e->stream)) return fail(SBK_EHIP, "commit launch failed");
+    return 0;
+}
+
+int run_attempts(sbk_engine *e, int count)
+{
+    if (e->cfg.use_graph && !e->cfg.timing) {
+        if (!e->graph_exec) {
+            // build the list before capture: allocation is not allowed while capturing
+            ItemList *L = nullptr;
+            int rc = get_list(e, e->cfg.precision, true, &L);
+            if (rc) return rc;
+            HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_attempt(e);
+            hipGraph_t g = nullptr;
+            hipError_t ce = hipStreamEndCapture(e->stream, &g);
+            if (rc) {
+                if (g) hipGraphDestroy(g);
+                return rc;
+            }
+            if (ce != hipSuccess) return fail(SBK_EHIP, "graph capture failed: %s", hipGetErrorString(ce));
+            e->graph = g;
+            HIP_TRY(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0));
+        }
+        for (int i = 0; i < count; ++i) HIP_TRY(hipGraphLaunch(e->graph_exec, e->stream));
+        return 0;
+    }
+    for (int i = 0; i < count; ++i) {
+        int rc = enqueue_attempt(e);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int read_states(sbk_engine *e)
+{
+    HIP_TRY(hipMemcpyAsync(e->h_state, e->d_state, sizeof(sbk::CloudState) * e->clouds.size(),
+                           hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return harvest_events(e);
+}
+
+int prepare(sbk_engine *e)
+{
+    if (!e) return fail(SBK_EINVAL, "null engine");
+    HIP_TRY(hipSetDevice(e->device));
+    return finalize(e);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sbk_last_error(void)
+{
+    return g_err.c_str();
+}
+
+void sbk_default_config(sbk_config *c)
+{
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->precision = SBK_MIXED;
+    c->time_mode = SBK_TIME_FROZEN;
+    c->adaptive = 1;
+    c->use_graph = 1;
+    c->timing = 0;
+    c->rtol = 1e-7;   // bin/ode.cpp:515-516
+    c->atol = 1e-7;
+    c->h0 = 1e-9;     // bin/ode.cpp:509
+}
+
+void sbk_default_fields(sbk_fields *f)
+{
+    if (!f) return;
+    std::memset(f, 0, sizeof(*f));
+    // bin/trapparameters.cpp:2-10 and bin/ode.cpp:646-653
+    f->trap = 1;
+    f->kappa = 0.001;
+    f->Vdc = 5;
+    f->Vrf = 50;
+    f->r0 = 0.04;
+    f->z0 = 0.04;
+    f->freq_rf = 50e5;
+    f->coulomb = 0;
+    f->coulomb_scale = 1.0;
+    f->rq = 1.0;
+}
+
+int sbk_device_info(char *name, int name_len, int *cus, int *clock_mhz)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, dev));
+    if (name && name_len > 0) {
+        std::snprintf(name, name_len, "%s", p.gcnArchName[0] ? p.gcnArchName : p.name);
+    }
+    if (cus) *cus = p.multiProcessorCount;
+    if (clock_mhz) *clock_mhz = p.clockRate / 1000;
+    return 0;
+}
+
+int sbk_create(const sbk_config *cfg, sbk_engine **out)
+{
+    if (!cfg || !out) return fail(SBK_EINVAL, "null argument");
+    if (cfg->precision < SBK_FP64_EXACT || cfg->precision > SBK_MIXED_KAHAN)
+        return fail(SBK_EINVAL, "unknown precision %d", cfg->precision);
+    if (cfg->time_mode != SBK_TIME_FROZEN && cfg->time_mode != SBK_TIME_STAGE)
+        return fail(SBK_EINVAL, "unknown time mode %d", cfg->time_mode);
+    if (!(cfg->h0 > 0.0) || !(cfg->rtol >= 0.0) || !(cfg->atol >= 0.0) ||
+        (cfg->rtol == 0.0 && cfg->atol == 0.0))
+        return fail(SBK_EINVAL, "h0 must be > 0 and rtol/atol >= 0, not both 0");
+    std::unique_ptr<sbk_engine> e(new sbk_engine);
+    e->cfg = *cfg;
+    HIP_TRY(hipGetDevice(&e->device));
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, e->device));
+    e->cus = std::max(1, p.multiProcessorCount);
+    HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    *out = e.release();
+    return 0;
+}
+
+int sbk_destroy(sbk_engine *e)
+{
+    if (!e) return 0;
+    hipSetDevice(e->device);
+    if (e->stream) hipStreamSynchronize(e->stream);
+    free_lists(e);
+    for (auto &p : e->ev) {
+        hipEventDestroy(p.first);
+        hipEventDestroy(p.second);
+    }
+    hipFree(e->d_f64);
+    hipFree(e->d_qf);
+    hipFree(e->d_cloud_of);
+    hipFree(e->d_cp);
+    hipFree(e->d_params);
+    hipFree(e->d_state);
+    if (e->h_state) hipHostFree(e->h_state);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+    return 0;
+}
+
+int sbk_add_cloud(sbk_engine *e, int n, const double *pos, const double *vel, const double *mass,
+                  const int *charge, const sbk_fields *f, double lifetime, int *cloud_id)
+{
+    if (!e || !f) return fail(SBK_EINVAL, "null argument");
+    if (e->finalized) return fail(SBK_ESTATE, "clouds must be added before the first run");
+    if (n < 0 || (n > 0 && (!pos || !vel || !mass || !charge)))
+        return fail(SBK_EINVAL, "bad cloud arrays (n=%d)", n);
+    if (e->ntot + (long long)n > (1LL << 30)) return fail(SBK_EINVAL, "too many ions");
+    HostCloud c;
+    c.n = n;
+    c.f = *f;
+    c.lifetime = lifetime;
+    c.pos.assign(pos, pos + 3 * (size_t)n);
+    c.vel.assign(vel, vel + 3 * (size_t)n);
+    c.mass.assign(mass, mass + n);
+    c.charge.assign(charge, charge + n);
+    for (int i = 0; i < n; ++i)
+        if (!(c.mass[i] > 0.0)) return fail(SBK_EINVAL, "ion %d has mass <= 0", i);
+    e->clouds.push_back(std::move(c));
+    e->ntot += n;
+    if (cloud_id) *cloud_id = (int)e->clouds.size() - 1;
+    return 0;
+}
+
+int sbk_set_fields(sbk_engine *e, int cloud, const sbk_fields *f)
+{
+    if (!e || !f || cloud < 0 || cloud >= (int)e->clouds.size())
+        return fail(SBK_EINVAL, "bad cloud id");
+    HostCloud &c = e->clouds[cloud];
+    const bool coulomb_changed = (c.f.coulomb != 0) != (f->coulomb != 0);
+    const bool scale_changed = c.f.coulomb_scale != f->coulomb_scale;
+    c.f = *f;
+    if (!e->finalized) return 0;
+    HIP_TRY(hipSetDevice(e->device));
+    sbk::CloudParams p = make_params(c.f, c.off, c.n);
+    HIP_TRY(hipMemcpyAsync(e->d_params + cloud, &p, sizeof(p), hipMemcpyHostToDevice, e->stream));
+    if (scale_changed && c.n > 0) {
+        std::vector<double> qm(c.n), kc(c.n), qd(c.n);
+        std::vector<float> qf(c.n);
+        ion_constants(c, qm.data(), kc.data(), qd.data(), qf.data());
+        HIP_TRY(hipMemcpyAsync(e->d_kc + c.off, kc.data(), sizeof(double) * c.n,
+                               hipMemcpyHostToDevice, e->stream));
+    }
+    if (coulomb_changed) free_lists(e);
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int sbk_set_step(sbk_engine *e, int cloud, double h)
+{
+    int rc = prepare(e);
+    if (rc) return rc;
+    if (cloud < 0 || cloud >= (int)e->clouds.size() || !(h > 0.0))
+        return fail(SBK_EINVAL, "bad cloud id or step");
+    rc = read_states(e);
+    if (rc) return rc;
+    sbk::CloudState s = e->h_state[cloud];
+    s.h = h;
+    s.hnext = h;
+    HIP_TRY(hipMemcpyAsync(e->d_state + cloud, &s, sizeof(s), hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int sbk_attempts(sbk_engine *e, int count)
+{
+    int rc = prepare(e);
+    if (rc) return rc;
+    return run_attempts(e, count);
+}
+
+int sbk_synchronize(sbk_engine *e)
+{
+    int rc = prepare(e);
+    if (rc) return rc;
+    return read_states(e);
+}
+
+static int check_errors(sbk_engine *e)
+{
+    for (size_t i = 0; i < e->clouds.size(); ++i)
+        if (e->h_state[i].error)
+            return fail(SBK_EUNDERFLOW, "stepsize underflow created (cloud %zu)", i);
+    return 0;
+}
+
+int sbk_step(sbk_engine *e)
+{
+    int rc = prepare(e);
+    if (rc) return rc;
+    if (sbk::launch_begin(e->dev, INFINITY, 1, 1, e->stream))
+        return fail(SBK_EHIP, "begin kernel launch failed");
+    for (long long guard = 0; guard < 100000; ++guard) {
+        rc = run_attempts(e, 1);
+        if (rc) return rc;
+        rc = read_states(e);
+        if (rc) return rc;
+        rc = check_errors(e);
+        if (rc) return rc;
+        bool any = false;
+        for (size_t i = 0; i < e->clouds.size(); ++i) any |= e->h_state[i].active != 0;
+        if (!any) return 0;
+    }
+    return fail(SBK_ESTATE, "no accepted step after 100000 attempts");
+}
+
+int sbk_advance(sbk_engine *e, double duration, long long max_attempts)
+{
+    int rc = prepare(e);
+    if (rc) return rc;
+    if (!(duration >= 0.0)) return fail(SBK_EINVAL, "duration must be >= 0");
+    if (sbk::launch_begin(e->dev, duration, -1, 1, e->stream))
+        return fail(SBK_EHIP, "begin kernel launch failed");
+    long long launched = 0;
+    int batch = 4;
+    while (launched < max_attempts) {
+        const int k = (int)std::min<long long>(batch, max_attempts - launched);
+        rc = run_attempts(e, k);
+        if (rc) return rc;
+        launched += k;
+        rc = read_states(e);
+        if (rc) return rc;
+        rc = check_errors(e);
+        if (rc) return rc;
+        bool any = false;
+        for (size_t i = 0; i < e->clouds.size(); ++i) any |= e->h_state[i].active != 0;
+        if (!any) break;
+        batch = std::min(batch * 2, 64);
+    }
+    return 0;
+}
+
+int sbk_get_state(sbk_engine *e, int cloud, double *pos, double *vel)
+{
+    int rc = prepare(e);
+    if (rc) return rc;
+    if (cloud < 0 || cloud >= (int)e->clouds.size()) return fail(SBK_EINVAL, "bad cloud id");
+    const HostCloud &c = e->clouds[cloud];
+    std::vector<double> buf(6 * (size_t)c.n);
+    for (int k = 0; k < 3; ++k) {
+        HIP_TRY(hipMemcpyAsync(buf.data() + k * (size_t)c.n, e->dev.P0[k] + c.off,
+                               sizeof(double) * c.n, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipMemcpyAsync(buf.data() + (3 + k) * (size_t)c.n, e->dev.V0[k] + c.off,
+                               sizeof(double) * c.n, hipMemcpyDeviceToHost, e->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (int i = 0; i < c.n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            if (pos) pos[3 * i + k] = buf[k * (size_t)c.n + i];
+            if (vel) vel[3 * i + k] = buf[(3 + k) * (size_t)c.n + i];
+        }
+    return 0;
+}
+
+int sbk_get_status(sbk_engine *e, int cloud, sbk_cloud_status *st)
+{
+    int rc = prepare(e);
+    if (rc) return rc;
+    if (!st || cloud < 0 || cloud >= (int)e->clouds.size()) return fail(SBK_EINVAL, "bad cloud id");
+    rc = read_states(e);
+    if (rc) return rc;
+    const sbk::CloudState &s = e->h_state[cloud];
+    st->lifetime = s.t;
+    st->h = s.h;
+    st->hnext = s.hnext;
+    st->errold = s.errold;
+    st->last_err = s.last_err;
+    st->t_end = s.t_end;
+    st->reject = s.reject;
+    st->active = s.active;
+    st->accepted_last = s.accepted_last;
+    st->error = s.error;
+    st->steps = s.steps;
+    st->attempts = s.attempts;
+    st->n = e->clouds[cloud].n;
+    ItemList *L = nullptr;
+    st->slices = (get_list(e, e->cfg.precision, true, &L) == 0) ? L->slices[cloud] : 0;
+    return 0;
+}
+
+int sbk_get_stats(sbk_engine *e, sbk_stats *out)
+{
+    int rc = prepare(e);
+    if (rc) return rc;
+    if (!out) return fail(SBK_EINVAL, "null stats");
+    rc = read_states(e);
+    if (rc) return rc;
+    std::memset(out, 0, sizeof(*out));
+    out->coulomb_ms = e->coulomb_ms;
+    out->coulomb_launches = e->coulomb_launches;
+    for (size_t i = 0; i < e->clouds.size(); ++i) {
+        const double n = e->clouds[i].n;
+        const long long att = e->h_state[i].attempts - e->attempts_at_reset[i];
+        const long long stp = e->h_state[i].steps - e->steps_at_reset[i];
+        out->attempts += att;
+        out->steps += stp;
+        out->ion_steps += (double)stp * n;
+        if (e->clouds[i].f.coulomb) out->pair_interactions += (double)att * 7.0 * n * (n - 1.0);
+    }
+    return 0;
+}
+
+int sbk_reset_stats(sbk_engine *e)
+{
+    int rc = prepare(e);
+    if (rc) return rc;
+    rc = read_states(e);
+    if (rc) return rc;
+    for (size_t i = 0; i < e->clouds.size(); ++i) {
+        e->attempts_at_reset[i] = e->h_state[i].attempts;
+        e->steps_at_reset[i] = e->h_state[i].steps;
+    }
+    e->coulomb_ms = 0.0;
+    e->coulomb_launches = 0;
+    return 0;
+}
+
+static int eval_common(sbk_engine *e, int cloud, int precision, int with_fields, double *out)
+{
+    int rc = prepare(e);
+    if (rc) return rc;
+    if (!out || cloud < 0 || cloud >= (int)e->clouds.size()) return fail(SBK_EINVAL, "bad cloud id");
+    if (precision < SBK_FP64_EXACT || precision > SBK_MIXED_KAHAN)
+        return fail(SBK_EINVAL, "unknown precision %d", precision);
+    const HostCloud &c = e->clouds[cloud];
+    ItemList *L = nullptr;
+    rc = get_list(e, precision, false, &L);
+    if (rc) return rc;
+    sbk::EngineDev d = e->dev;
+    d.slices = L->d_slices;
+    const bool need = !with_fields || c.f.coulomb;
+    if (need && L->nitems > 0) {
+        sbk::CoulombArgs a = coulomb_args(e, *L, d.P0, false);
+        rc = launch_coulomb_timed(e, precision, a);
+        if (rc) return rc;
+    }
+    if (sbk::launch_eval_derivs(d, with_fields, e->stream))
+        return fail(SBK_EHIP, "derivs kernel launch failed");
+    std::vector<double> buf(3 * (size_t)c.n);
+    for (int k = 0; k < 3; ++k)
+        HIP_TRY(hipMemcpyAsync(buf.data() + k * (size_t)c.n, d.derivs_out + k * e->stride + c.off,
+                               sizeof(double) * c.n, hipMemcpyDeviceToHost, e->stream));
+    rc = read_states(e);
+    if (rc) return rc;
+    for (int i = 0; i < c.n; ++i)
+        for (int k = 0; k < 3; ++k) out[3 * i + k] = buf[k * (size_t)c.n + i];
+    return 0;
+}
+
+int sbk_eval_force(sbk_engine *e, int cloud, double *derivs)
+{
+    if (!e) return fail(SBK_EINVAL, "null engine");
+    return eval_common(e, cloud, e->cfg.precision, 1, derivs);
+}
+
+int sbk_eval_coulomb(sbk_engine *e, int cloud, int precision, double *out)
+{
+    return eval_common(e, cloud, precision, 0, out);
+}
+
+int sbk_set_launch(sbk_engine *e, int tpt, int slices)
+{
+    if (!e) return fail(SBK_EINVAL, "null engine");
+    if (tpt != 1 && tpt != 2 && tpt != 4) return fail(SBK_EINVAL, "targets_per_thread must be 1, 2 or 4");
+    if (slices < 0) return fail(SBK_EINVAL, "slices must be >= 0");
+    if (tpt != e->tpt || slices != e->slices_req) {
+        e->tpt = tpt;
+        e->slices_req = slices;
+        destroy_graph(e);
+    }
+    return 0;
+}
+
+int sbk_time_coulomb(sbk_engine *e, int precision, int reps, double *ms_per_launch)
+{
+    int rc = prepare(e);
+    if (rc) return rc;
+    if (reps <= 0 || !ms_per_launch) return fail(SBK_EINVAL, "reps must be > 0");
+    if (precision < SBK_FP64_EXACT || precision > SBK_MIXED_KAHAN)
+        return fail(SBK_EINVAL, "unknown precision %d", precision);
+    ItemList *L = nullptr;
+    rc = get_list(e, precision, false, &L);
+    if (rc) return rc;
+    sbk::CoulombArgs a = coulomb_args(e, *L, e->dev.P0, false);
+    if (precision == SBK_FP64) a.eps_d = 1e-40;
+    hipEvent_t t0, t1;
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    HIP_TRY(hipEventRecord(t0, e->stream));
+    for (int i = 0; i < reps; ++i)
+        if (sbk::launch_coulomb(kp(precision), e->tpt, a, e->stream))
+            return fail(SBK_EHIP, "Coulomb kernel launch failed");
+    HIP_TRY(hipEventRecord(t1, e->stream));
+    HIP_TRY(hipEventSynchronize(t1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    *ms_per_launch = ms / reps;
+    return 0;
+}
+
+int sbk_autotune(sbk_engine *e, int precision, int *tpt_out, int *slices_out, const char *table)
+{
+    int rc = prepare(e);
+    if (rc) return rc;
+    if (precision == SBK_FP64_EXACT) {
+        if (tpt_out) *tpt_out = 1;
+        if (slices_out) *slices_out = 1;
+        return 0;
+    }
+    const int tpts[3] = {1, 2, 4};
+    const int smul[6] = {0, 1, 2, 4, 8, 16};
+    int best_t = e->tpt, best_s = e->slices_req;
+    double best = 1e300;
+    char dev_name[128] = {0};
+    sbk_device_info(dev_name, sizeof(dev_name), nullptr, nullptr);
+    FILE *fp = table ? std::fopen(table, "a") : nullptr;
+    const int old_t = e->tpt, old_s = e->slices_req;
+    for (int t : tpts) {
+        for (int m : smul) {
+            int slices = 0;
+            if (m > 0) {
+                // explicit slice count: m x the automatic one at this shape
+                e->tpt = t;
+                e->slices_req = 0;
+                ItemList *L = nullptr;
+                rc = get_list(e, precision, false, &L);
+                if (rc) break;
+                int base = 1;
+                for (int s : L->slices) base = std::max(base, s);
+                slices = std::max(1, base * m / 4);
+            }
+            e->tpt = t;
+            e->slices_req = slices;
+            double ms = 0.0;
+            rc = sbk_time_coulomb(e, precision, 1, &ms);   // warm-up and list build
+            if (rc) break;
+            rc = sbk_time_coulomb(e, precision, 3, &ms);
+            if (rc) break;
+            if (fp)
+                std::fprintf(fp, "%s\t%d\t%zu\t%d\t%d\t%d\t%.6f\n", dev_name, e->ntot,
+                             e->clouds.size(), precision, t, slices, ms);
+            if (ms < best) {
+                best = ms;
+                best_t = t;
+                best_s = slices;
+            }
+        }
+        if (rc) break;
+    }
+    if (fp) std::fclose(fp);
+    e->tpt = old_t;
+    e->slices_req = old_s;
+    if (rc) return rc;
+    rc = sbk_set_launch(e, best_t, best_s);
+    if (rc) return rc;
+    if (tpt_out) *tpt_out = best_t;
+    if (slices_out) *slices_out = best_s;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+// sbk_internal.h -- device data layout shared by the HIP kernels
+// (sbk_kernels.hip) and the host engine (sbk_engine.cpp).
+//
+// HBM layout of an engine holding clouds c = 0..C-1 with n_c ions each,
+// concatenated at offsets off_c (global ion index g):
+//   P0[3][stride]    positions at t_n        (IonCloud::pos,  bin/ioncloud.h:27)
+//   V0[3][stride]    velocities at t_n       (IonCloud::vel,  bin/ioncloud.h:29)
+//   PS[3][stride]    stage positions         (IonCloud::pos2, bin/ioncloud.h:28)
+//   KV[s][3][stride] stage velocities, s=2..7 (k_s[0..2] of bin/ode.cpp:110-173;
+//                    KV[1] aliases V0, and KV[7] is the new velocity vel2)
+//   KA[s][3][stride] stage accelerations, s=1..7 (k_s[3..5], bin/ode.cpp:104-180)
+//   CP[slice][3][stride] Coulomb partial sums per source slice
+//   ERR[stride]      per-ion squared scaled error of the embedded pair
+// All FP64, structure of arrays, so every per-ion kernel reads and writes
+// unit-stride, fully coalesced lines.
+#ifndef SBK_INTERNAL_H
+#define SBK_INTERNAL_H
+
+#include <stdint.h>
+
+namespace sbk {
+
+// Coulomb tile width: sources staged in LDS per pass = threads per block.
+constexpr int kBlock = 256;
+// Sources accumulated in FP32 before folding into the cross-tile sum.
+constexpr int kSubTile = 64;
+
+struct CloudParams {
+    // reference trap term (bin/force.cpp:40-44,70-80)
+    int trap, tof, coulomb, ramp;
+    double kappa, Vdc, Vrf, r02, z02, omega_rf, E_kick;
+    double ramp_t0, ramp_t1, newVrf, newVdc;
+    // extensions
+    double cpen;              // V0 / (2 d0^2)
+    double Bz;
+    double Ed, fd, phid;
+    double Uq_rq2, fq, phiq;  // Uq / rq^2
+    double delta;
+    // layout
+    int off, n, slices, pad;
+};
+
+struct CloudState {
+    double t, h, hnext, errold, last_err, t_end;
+    int reject, active, accepted_last, error;
+    long long steps, attempts;
+    int budget;               // accepted steps still wanted (-1: unlimited)
+    int pad;
+};
+
+// One Coulomb work item: targets [i0,i1) against sources [j0,j1) of one
+// cloud, written into partial-sum slice `slice`.
+struct CoulombItem {
+    int cloud, i0, i1, j0, j1, slice, pad0, pad1;
+};
+
+struct CoulombArgs {
+    const double *tx, *ty, *tz;   // targets
+    const double *sx, *sy, *sz;   // sources
+    const double *sqd;            // source weight (charge), FP64
+    const float *sqf;             // source weight, FP32
+    const double *kc;             // per-target prefactor (nullptr: 1)
+    double *out;                  // out[slice*3*stride + comp*stride + g]
+    long long stride;
+    const CoulombItem *items;
+    int nitems;
+    int self_skip;                // FP64-exact: skip source index == target index
+    const CloudState *state;      // nullptr: every item active
+    float eps_f;
+    double eps_d;
+};
+
+struct EngineDev {
+    double *P0[3], *V0[3], *PS[3];
+    double *KV[8][3];
+    double *KA[8][3];
+    double *CP;
+    double *ERR;
+    double *derivs_out;           // [3][stride] scratch for sbk_eval_force/coulomb
+    long long stride;
+    const double *qm;             // charge*e/mass (bin/force.cpp:71)
+    const int *cloud_of;
+    const CloudParams *params;
+    CloudState *state;
+    const int *slices;            // per-cloud slice count of the current item list
+    int ntot, nclouds;
+    int time_mode, adaptive;
+    double rtol, atol;
+};
+
+// host-side launchers (sbk_kernels.hip)
+enum KernelPrecision { KP_FP64_EXACT = 0, KP_FP64 = 1, KP_MIXED = 2, KP_MIXED_KAHAN = 3 };
+
+int launch_coulomb(int precision, int targets_per_thread, const CoulombArgs &a,
+                   void *stream);
+int launch_stage(int stage, const EngineDev &d, void *stream);
+int launch_controller(const EngineDev &d, void *stream);
+int launch_commit(const EngineDev &d, void *stream);
+int launch_begin(const EngineDev &d, double duration, int budget, int set_end, void *stream);
+int launch_eval_derivs(const EngineDev &d, int with_fields, void *stream);
+
+}  // namespace sbk
+
+#endif

@@ -1,0 +1,595 @@
+// sbk_kernels.hip -- CDNA4 (gfx950) kernels of the Simbuca force and
+// Dormand-Prince 5(4) engine.
+//
+// Built with -ffp-contract=off: every fused multiply-add below is written
+// out with fma()/fmaf(), everything else rounds exactly as written, which is
+// what lets the FP64-exact Coulomb sum and the stage combinations reproduce
+// the reference's CPU arithmetic (bin/force.cpp:34-175, bin/ode.cpp:61-218)
+// bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <float.h>
+#include <math.h>
+
+#include "sbk_internal.h"
+
+namespace sbk {
+
+// bin/globals.h:6
+constexpr double kPi = 3.14159265358979323;
+
+// ---------------------------------------------------------------------------
+// Work-group -> work-item mapping.  Work-groups are dealt round-robin to the
+// 8 XCDs; this gives each XCD one contiguous run of items, so the items of
+// one source slice (which reuse the same sources) share an L2.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int xcd_item(int b, int nb)
+{
+    const int xcd = b & 7, idx = b >> 3;
+    const int q = nb >> 3, r = nb & 7;
+    return xcd * q + (xcd < r ? xcd : r) + idx;
+}
+
+// ---------------------------------------------------------------------------
+// Coulomb direct sum, FP64 in the reference's per-target order.
+// For target g: f = sum over sources s != g, in increasing s, of
+// ((r*invDist^3)*k_g)*q_s with r = x_s - x_g, invDist = 1/sqrt(|r|^2)
+// (bin/force.cpp:158-170).  One thread per target, no source split.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void coulomb_fp64_exact(CoulombArgs a)
+{
+    __shared__ double lx[kBlock], ly[kBlock], lz[kBlock], lq[kBlock];
+    const CoulombItem it = a.items[xcd_item(blockIdx.x, gridDim.x)];
+    if (a.state && !a.state[it.cloud].active) return;
+    const int tid = threadIdx.x;
+    const int g = it.i0 + tid;
+    const bool valid = g < it.i1;
+    const double bx = valid ? a.tx[g] : 0.0;
+    const double by = valid ? a.ty[g] : 0.0;
+    const double bz = valid ? a.tz[g] : 0.0;
+    const double k = (valid && a.kc) ? a.kc[g] : 1.0;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int t0 = it.j0; t0 < it.j1; t0 += kBlock) {
+        const int cnt = min(kBlock, it.j1 - t0);
+        __syncthreads();
+        if (tid < cnt) {
+            lx[tid] = a.sx[t0 + tid];
+            ly[tid] = a.sy[t0 + tid];
+            lz[tid] = a.sz[t0 + tid];
+            lq[tid] = a.sqd[t0 + tid];
+        }
+        __syncthreads();
+        for (int j = 0; j < cnt; ++j) {
+            if (a.self_skip && t0 + j == g) continue;
+            const double rx = lx[j] - bx;
+            const double ry = ly[j] - by;
+            const double rz = lz[j] - bz;
+            const double distSqr = rx * rx + ry * ry + rz * rz + a.eps_d;
+            const double invDist = 1. / sqrt(distSqr);
+            const double invDistCube = (invDist * invDist * invDist);
+            fx += rx * invDistCube * k * lq[j];
+            fy += ry * invDistCube * k * lq[j];
+            fz += rz * invDistCube * k * lq[j];
+        }
+    }
+    if (valid) {
+        double *o = a.out + (long long)it.slice * 3 * a.stride + g;
+        o[0] = fx;
+        o[a.stride] = fy;
+        o[2 * a.stride] = fz;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Coulomb direct sum, tiled.  A work-group owns kBlock*R targets (R per
+// thread, strided by kBlock so a wave's loads are unit stride) and one slice
+// of the sources.  Sources pass through LDS kBlock at a time; inside a tile
+// every lane reads the same source (an LDS broadcast) and updates its R
+// targets.
+//
+// MIXED: separations are formed in FP64 and rounded to FP32; |r|^2, rsqrt
+// and the sum over kSubTile sources are FP32; the sub-tile sums are folded
+// into FP64 (KAHAN: compensated FP32) accumulators.
+// FP64: the same tiling in FP64 with 1/sqrt.
+// The self pair has r = 0 and contributes exactly 0 through the eps term.
+// ---------------------------------------------------------------------------
+template <int R, int MODE>
+__global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
+{
+    __shared__ double lx[kBlock], ly[kBlock], lz[kBlock];
+    __shared__ float lqf[kBlock];
+    __shared__ double lqd[(MODE == KP_FP64) ? kBlock : 1];
+
+    const CoulombItem it = a.items[xcd_item(blockIdx.x, gridDim.x)];
+    if (a.state && !a.state[it.cloud].active) return;
+    const int tid = threadIdx.x;
+
+    double px[R], py[R], pz[R];
+    double Ax[R], Ay[R], Az[R];
+    float Sx[R], Sy[R], Sz[R], Cx[R], Cy[R], Cz[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int g = it.i0 + r * kBlock + tid;
+        const bool v = g < it.i1;
+        px[r] = v ? a.tx[g] : 0.0;
+        py[r] = v ? a.ty[g] : 0.0;
+        pz[r] = v ? a.tz[g] : 0.0;
+        Ax[r] = Ay[r] = Az[r] = 0.0;
+        Sx[r] = Sy[r] = Sz[r] = 0.f;
+        Cx[r] = Cy[r] = Cz[r] = 0.f;
+    }
+    const float epsf = a.eps_f;
+    const double epsd = a.eps_d;
+
+    for (int t0 = it.j0; t0 < it.j1; t0 += kBlock) {
+        const int cnt = min(kBlock, it.j1 - t0);
+        __syncthreads();
+        if (tid < cnt) {
+            lx[tid] = a.sx[t0 + tid];
+            ly[tid] = a.sy[t0 + tid];
+            lz[tid] = a.sz[t0 + tid];
+            if (MODE == KP_FP64)
+                lqd[tid] = a.sqd[t0 + tid];
+            else
+                lqf[tid] = a.sqf[t0 + tid];
+        }
+        __syncthreads();
+
+        if (MODE == KP_FP64) {
+            for (int j = 0; j < cnt; ++j) {
+                const double sx = lx[j], sy = ly[j], sz = lz[j], q = lqd[j];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const double dx = sx - px[r], dy = sy - py[r], dz = sz - pz[r];
+                    const double d2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, epsd)));
+                    const double inv = 1.0 / sqrt(d2);
+                    const double w = q * (inv * inv) * inv;
+                    Ax[r] = fma(dx, w, Ax[r]);
+                    Ay[r] = fma(dy, w, Ay[r]);
+                    Az[r] = fma(dz, w, Az[r]);
+                }
+            }
+        } else {
+            for (int jj = 0; jj < cnt; jj += kSubTile) {
+                float ax[R], ay[R], az[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) ax[r] = ay[r] = az[r] = 0.f;
+                const int m = min(kSubTile, cnt - jj);
+                auto body = [&](int j) {
+                    const double sx = lx[j], sy = ly[j], sz = lz[j];
+                    const float q = lqf[j];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const float dx = (float)(sx - px[r]);
+                        const float dy = (float)(sy - py[r]);
+                        const float dz = (float)(sz - pz[r]);
+                        float d2 = fmaf(dz, dz, epsf);
+                        d2 = fmaf(dy, dy, d2);
+                        d2 = fmaf(dx, dx, d2);
+                        const float inv = __builtin_amdgcn_rsqf(d2);
+                        const float w = (q * inv) * (inv * inv);
+                        ax[r] = fmaf(dx, w, ax[r]);
+                        ay[r] = fmaf(dy, w, ay[r]);
+                        az[r] = fmaf(dz, w, az[r]);
+                    }
+                };
+                if (m == kSubTile) {
+#pragma unroll 8
+                    for (int j = 0; j < kSubTile; ++j) body(jj + j);
+                } else {
+                    for (int j = 0; j < m; ++j) body(jj + j);
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (MODE == KP_MIXED_KAHAN) {
+                        float y, t;
+                        y = ax[r] - Cx[r]; t = Sx[r] + y; Cx[r] = (t - Sx[r]) - y; Sx[r] = t;
+                        y = ay[r] - Cy[r]; t = Sy[r] + y; Cy[r] = (t - Sy[r]) - y; Sy[r] = t;
+                        y = az[r] - Cz[r]; t = Sz[r] + y; Cz[r] = (t - Sz[r]) - y; Sz[r] = t;
+                    } else {
+                        Ax[r] += (double)ax[r];
+                        Ay[r] += (double)ay[r];
+                        Az[r] += (double)az[r];
+                    }
+                }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int g = it.i0 + r * kBlock + tid;
+        if (g < it.i1) {
+            if (MODE == KP_MIXED_KAHAN) {
+                Ax[r] = (double)Sx[r] - (double)Cx[r];
+                Ay[r] = (double)Sy[r] - (double)Cy[r];
+                Az[r] = (double)Sz[r] - (double)Cz[r];
+            }
+            const double k = a.kc ? a.kc[g] : 1.0;
+            double *o = a.out + (long long)it.slice * 3 * a.stride + g;
+            o[0] = k * Ax[r];
+            o[a.stride] = k * Ay[r];
+            o[2 * a.stride] = k * Az[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dormand-Prince 5(4) tableau, bin/ode.cpp:76-86.  kA[s][j] is the
+// coefficient of k_j in the input of stage s (kA[7] = b), kE the error
+// weights e = b - b*.  Zero entries are skipped, as the reference's
+// expressions omit them.
+// ---------------------------------------------------------------------------
+__device__ constexpr double kC[8] = {0, 0, 1.0 / 5.0, 3.0 / 10.0, 4.0 / 5.0, 8.0 / 9.0, 1.0, 1.0};
+__device__ constexpr double kA[8][8] = {
+    {0}, {0},
+    {0, 1.0 / 5.0},
+    {0, 3.0 / 40.0, 9.0 / 40.0},
+    {0, 44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0},
+    {0, 19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0},
+    {0, 9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0},
+    {0, 35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0},
+};
+__device__ constexpr double kE[8] = {0, 71.0 / 57600.0, 0.0, -71.0 / 16695.0, 71.0 / 1920.0,
+                                     -17253.0 / 339200.0, 22.0 / 525.0, -1.0 / 40.0};
+
+// Sum_{j=1..S} coef[j]*k_j evaluated left to right over the non-zero terms,
+// where k_S is `own` and k_j (j<S) is read from K[j][comp][g].
+template <int S, int NEXT>
+__device__ __forceinline__ double combo(double *const (*K)[3], int comp, long long g, double own)
+{
+    double acc = 0.0;
+    bool first = true;
+#pragma unroll
+    for (int j = 1; j <= S; ++j) {
+        const double c = kA[NEXT][j];
+        if (c == 0.0) continue;
+        const double kj = (j == S) ? own : K[j][comp][g];
+        const double term = c * kj;
+        acc = first ? term : acc + term;
+        first = false;
+    }
+    return acc;
+}
+
+// force() at one ion (bin/force.cpp:58-105 order), then the extension terms,
+// then the Coulomb subtraction.  t_life is IonCloud::lifetime (drives the
+// trap ramp, bin/ion_fly.cpp:235-239); ts is the field evaluation time.
+__device__ __forceinline__ void ion_accel(const CloudParams &p, double t_life, double ts,
+                                          double qm, double x, double y, double z, double vx,
+                                          double vy, double vz, double Fx, double Fy, double Fz,
+                                          double &ax, double &ay, double &az)
+{
+    ax = 0;
+    ay = 0;
+    az = 0;
+    if (p.trap) {
+        double Vrf = p.Vrf, Vdc = p.Vdc;
+        if (p.ramp) {
+            const double ramp = (t_life - p.ramp_t0) / (p.ramp_t1 - p.ramp_t0);
+            Vrf = p.Vrf * (1 - ramp) + p.newVrf * ramp;
+            Vdc = p.Vdc * (1 - ramp) + p.newVdc * ramp;
+        }
+        const double kVdc = p.kappa * Vdc;
+        const double crf = cos(p.omega_rf * ts);
+        ax += qm * (kVdc / p.z02 - Vrf * crf / p.r02) * x;
+        ay += qm * (kVdc / p.z02 + Vrf * crf / p.r02) * y;
+        az += qm * (-2.0 * kVdc / p.z02) * z;
+    }
+    if (p.tof) ax += qm * p.E_kick;
+    if (p.cpen != 0.0) {
+        ax += qm * p.cpen * x;
+        ay += qm * p.cpen * y;
+        az += qm * (-2.0 * p.cpen) * z;
+    }
+    if (p.Bz != 0.0) {
+        ax += qm * p.Bz * vy;
+        ay -= qm * p.Bz * vx;
+    }
+    if (p.Ed != 0.0) ax += qm * (p.Ed * cos(2.0 * kPi * p.fd * ts + p.phid));
+    if (p.Uq_rq2 != 0.0) {
+        const double cq = p.Uq_rq2 * cos(2.0 * kPi * p.fq * ts + p.phiq);
+        ax += qm * cq * x;
+        ay -= qm * cq * y;
+    }
+    if (p.delta != 0.0) {
+        ax -= p.delta * vx;
+        ay -= p.delta * vy;
+        az -= p.delta * vz;
+    }
+    ax -= Fx;
+    ay -= Fy;
+    az -= Fz;
+}
+
+__device__ __forceinline__ void coulomb_total(const EngineDev &d, const CloudParams &p, int c,
+                                              long long g, double &Fx, double &Fy, double &Fz)
+{
+    Fx = Fy = Fz = 0.0;
+    if (!p.coulomb) return;
+    const int ns = d.slices[c];
+    const long long st = d.stride;
+    Fx = d.CP[g];
+    Fy = d.CP[st + g];
+    Fz = d.CP[2 * st + g];
+    for (int s = 1; s < ns; ++s) {
+        const double *cp = d.CP + (long long)s * 3 * st + g;
+        Fx += cp[0];
+        Fy += cp[st];
+        Fz += cp[2 * st];
+    }
+}
+
+// Stage S of one attempt for every active ion: derivatives at the stage
+// state, then the next stage state (bin/ode.cpp:103-175), or for S == 7 the
+// new state and its scaled error (bin/ode.cpp:177-205).
+template <int S>
+__global__ __launch_bounds__(256) void stage_kernel(EngineDev d)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= d.ntot) return;
+    const int c = d.cloud_of[g];
+    const CloudState &cs = d.state[c];
+    if (!cs.active) return;
+    const CloudParams &p = d.params[c];
+    const double h = cs.h, t = cs.t;
+    const double ts = d.time_mode ? t + kC[S] * h : t;
+
+    double *const *pos_s = (S == 1) ? d.P0 : d.PS;
+    const double x = pos_s[0][g], y = pos_s[1][g], z = pos_s[2][g];
+    const double vx = d.KV[S][0][g], vy = d.KV[S][1][g], vz = d.KV[S][2][g];
+    double Fx, Fy, Fz;
+    coulomb_total(d, p, c, g, Fx, Fy, Fz);
+    double a3[3];
+    ion_accel(p, t, ts, d.qm[g], x, y, z, vx, vy, vz, Fx, Fy, Fz, a3[0], a3[1], a3[2]);
+    const double v3[3] = {vx, vy, vz};
+    const double s3[3] = {x, y, z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d.KA[S][k][g] = a3[k];
+
+    if constexpr (S < 7) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            d.PS[k][g] = d.P0[k][g] + h * combo<S, S + 1>(d.KV, k, g, v3[k]);
+            d.KV[S + 1][k][g] = d.V0[k][g] + h * combo<S, S + 1>(d.KA, k, g, a3[k]);
+        }
+    } else {
+        // y_{n+1} equals the stage-7 state: a_7j = b_j, same evaluation order.
+        double err = 0.0;
+        bool first = true;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            double *const(*K)[3] = half ? d.KA : d.KV;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double own = half ? a3[k] : v3[k];
+                double acc = 0.0;
+                bool f1 = true;
+#pragma unroll
+                for (int j = 1; j <= 7; ++j) {
+                    if (kE[j] == 0.0) continue;
+                    const double kj = (j == 7) ? own : K[j][k][g];
+                    const double term = kE[j] * kj;
+                    acc = f1 ? term : acc + term;
+                    f1 = false;
+                }
+                const double yerr = h * acc;
+                const double old = half ? d.V0[k][g] : d.P0[k][g];
+                const double neu = half ? v3[k] : s3[k];
+                const double sk = d.atol + d.rtol * fmax(fabs(old), fabs(neu));
+                const double term = (yerr / sk) * (yerr / sk);
+                err = first ? term : err + term;
+                first = false;
+            }
+        }
+        d.ERR[g] = err;
+    }
+}
+
+// Per-cloud error norm (bin/ode.cpp:215), controller (bin/ode.cpp:426-457)
+// and the step bookkeeping of bin/ode.cpp:470-493.  One work-group per cloud;
+// the reduction order is fixed, so results are run-to-run reproducible.
+__global__ __launch_bounds__(256) void controller_kernel(EngineDev d)
+{
+    __shared__ double red[256];
+    const int c = blockIdx.x;
+    CloudState &cs = d.state[c];
+    const int active = cs.active;
+    if (!active) {
+        if (threadIdx.x == 0) cs.accepted_last = 0;
+        return;
+    }
+    const CloudParams &p = d.params[c];
+    double sum = 0.0;
+    for (int i = threadIdx.x; i < p.n; i += 256) sum += d.ERR[p.off + i];
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double err = sqrt(red[0] / (6 * p.n));
+    cs.last_err = err;
+    cs.attempts++;
+    int accept = 1;
+    if (d.adaptive) {
+        const double beta = 0.0, alpha = 0.2 - beta * 0.75, safe = 0.9, minscale = 0.2,
+                     maxscale = 10.0;
+        double scale;
+        if (err <= 1.0) {
+            if (err == 0.0)
+                scale = maxscale;
+            else {
+                scale = safe * pow(err, -alpha) * pow(cs.errold, beta);
+                if (scale < minscale) scale = minscale;
+                if (scale > maxscale) scale = maxscale;
+            }
+            if (cs.reject)
+                cs.hnext = cs.h * fmin(scale, 1.0);
+            else
+                cs.hnext = cs.h * scale;
+            cs.errold = fmax(err, 1.0e-4);
+            cs.reject = 0;
+        } else {
+            scale = fmax(safe * pow(err, -alpha), minscale);
+            cs.h *= scale;
+            cs.reject = 1;
+            accept = 0;
+            if (fabs(cs.h) <= fabs(d.P0[0][p.off]) * DBL_EPSILON) {
+                cs.error = 1;
+                cs.active = 0;
+            }
+        }
+    }
+    cs.accepted_last = accept;
+    if (accept) {
+        cs.t = cs.t + cs.h;
+        cs.h = cs.hnext;
+        cs.steps++;
+        if (cs.budget > 0) cs.budget--;
+        cs.active = (cs.t < cs.t_end) && cs.budget != 0;
+    }
+}
+
+// pos <- pos2, vel <- vel2 for clouds whose attempt was accepted
+// (bin/ode.cpp:488-489).
+__global__ __launch_bounds__(256) void commit_kernel(EngineDev d)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= d.ntot) return;
+    const int c = d.cloud_of[g];
+    if (!d.state[c].accepted_last) return;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        d.P0[k][g] = d.PS[k][g];
+        d.V0[k][g] = d.KV[7][k][g];
+    }
+}
+
+__global__ void begin_kernel(EngineDev d, double duration, int budget, int set_end)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.nclouds) return;
+    CloudState &cs = d.state[c];
+    if (set_end) cs.t_end = cs.t + duration;
+    cs.budget = budget;
+    cs.accepted_last = 0;
+    cs.active = (d.params[c].n > 0) && !cs.error && (cs.t < cs.t_end) && budget != 0;
+}
+
+// derivs_out = force() at (P0, V0, lifetime) when with_fields, else the
+// Coulomb sum f alone.
+__global__ __launch_bounds__(256) void eval_derivs_kernel(EngineDev d, int with_fields)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= d.ntot) return;
+    const int c = d.cloud_of[g];
+    const CloudParams &p = d.params[c];
+    double Fx, Fy, Fz;
+    if (with_fields) {
+        coulomb_total(d, p, c, g, Fx, Fy, Fz);
+        const double t = d.state[c].t;
+        double ax, ay, az;
+        ion_accel(p, t, t, d.qm[g], d.P0[0][g], d.P0[1][g], d.P0[2][g], d.V0[0][g],
+                  d.V0[1][g], d.V0[2][g], Fx, Fy, Fz, ax, ay, az);
+        Fx = ax; Fy = ay; Fz = az;
+    } else {
+        CloudParams q = p;
+        q.coulomb = 1;
+        coulomb_total(d, q, c, g, Fx, Fy, Fz);
+    }
+    d.derivs_out[g] = Fx;
+    d.derivs_out[d.stride + g] = Fy;
+    d.derivs_out[2 * d.stride + g] = Fz;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline int check_launch()
+{
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int R>
+static int launch_tiled(int precision, const CoulombArgs &a, hipStream_t s)
+{
+    switch (precision) {
+    case KP_FP64:
+        hipLaunchKernelGGL((coulomb_tiled<R, KP_FP64>), dim3(a.nitems), dim3(kBlock), 0, s, a);
+        break;
+    case KP_MIXED:
+        hipLaunchKernelGGL((coulomb_tiled<R, KP_MIXED>), dim3(a.nitems), dim3(kBlock), 0, s, a);
+        break;
+    case KP_MIXED_KAHAN:
+        hipLaunchKernelGGL((coulomb_tiled<R, KP_MIXED_KAHAN>), dim3(a.nitems), dim3(kBlock), 0,
+                           s, a);
+        break;
+    default:
+        return -1;
+    }
+    return check_launch();
+}
+
+int launch_coulomb(int precision, int tpt, const CoulombArgs &a, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (a.nitems <= 0) return 0;
+    if (precision == KP_FP64_EXACT) {
+        hipLaunchKernelGGL(coulomb_fp64_exact, dim3(a.nitems), dim3(kBlock), 0, s, a);
+        return check_launch();
+    }
+    switch (tpt) {
+    case 1: return launch_tiled<1>(precision, a, s);
+    case 2: return launch_tiled<2>(precision, a, s);
+    case 4: return launch_tiled<4>(precision, a, s);
+    default: return -1;
+    }
+}
+
+int launch_stage(int stage, const EngineDev &d, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 grid((unsigned)((d.ntot + 255) / 256)), block(256);
+    switch (stage) {
+    case 1: hipLaunchKernelGGL(stage_kernel<1>, grid, block, 0, s, d); break;
+    case 2: hipLaunchKernelGGL(stage_kernel<2>, grid, block, 0, s, d); break;
+    case 3: hipLaunchKernelGGL(stage_kernel<3>, grid, block, 0, s, d); break;
+    case 4: hipLaunchKernelGGL(stage_kernel<4>, grid, block, 0, s, d); break;
+    case 5: hipLaunchKernelGGL(stage_kernel<5>, grid, block, 0, s, d); break;
+    case 6: hipLaunchKernelGGL(stage_kernel<6>, grid, block, 0, s, d); break;
+    case 7: hipLaunchKernelGGL(stage_kernel<7>, grid, block, 0, s, d); break;
+    default: return -1;
+    }
+    return check_launch();
+}
+
+int launch_controller(const EngineDev &d, void *stream)
+{
+    hipLaunchKernelGGL(controller_kernel, dim3(d.nclouds), dim3(256), 0, (hipStream_t)stream, d);
+    return check_launch();
+}
+
+int launch_commit(const EngineDev &d, void *stream)
+{
+    hipLaunchKernelGGL(commit_kernel, dim3((unsigned)((d.ntot + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, d);
+    return check_launch();
+}
+
+int launch_begin(const EngineDev &d, double duration, int budget, int set_end, void *stream)
+{
+    hipLaunchKernelGGL(begin_kernel, dim3((d.nclouds + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, d, duration, budget, set_end);
+    return check_launch();
+}
+
+int launch_eval_derivs(const EngineDev &d, int with_fields, void *stream)
+{
+    hipLaunchKernelGGL(eval_derivs_kernel, dim3((unsigned)((d.ntot + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, d, with_fields);
+    return check_launch();
+}
+
+}  // namespace sbk

@@ -1,0 +1,84 @@
+"""Writes the golden fixtures of tests/golden/ by running the reference's own
+CPU force()/step() (oracle/_ref/ref_harness, compiled from the reference
+sources by `make -C oracle ref`).
+
+Each fixture holds the inputs and the reference's outputs as plain arrays
+(numpy .npz, no pickles).  Run from the repository root:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+TRAP_KEYS = ["trap", "kappa", "Vdc", "Vrf", "r0", "z0", "freq_rf", "tof", "E_kick", "coulomb",
+             "coulomb_scale"]
+
+
+def _fields_arr(f):
+    return np.array([float(getattr(f, k)) for k in TRAP_KEYS])
+
+
+def force_case(name, seed, n, sigma, species, charges, f, lifetime):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    amu = rng.choice(np.array(species, dtype=np.float64), n)
+    ch = rng.choice(np.array(charges, dtype=np.int32), n).astype(np.int32)
+    pos = rng.normal(0.0, sigma, (n, 3))
+    vel = rng.normal(0.0, 137.0, (n, 3))
+    out = oracle.ref_force(amu, ch, pos, vel, f, lifetime)
+    np.savez(os.path.join(OUT, name), mass_amu=amu, charge=ch, pos=pos, vel=vel,
+             fields=_fields_arr(f), lifetime=np.float64(lifetime), derivs=out)
+    print(name, n, float(np.abs(out).max()))
+
+
+def steps_case(name, seed, n, sigma, f, h0, rtol, atol, adaptive, nsteps, lifetime=0.0):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    amu = np.full(n, 133.0)
+    ch = np.ones(n, dtype=np.int32)
+    pos = rng.normal(0.0, sigma, (n, 3))
+    vel = rng.normal(0.0, 137.0, (n, 3))
+    p, v, sc, hist = oracle.ref_steps(amu, ch, pos, vel, f, lifetime, h0, rtol, atol, adaptive,
+                                      nsteps)
+    np.savez(os.path.join(OUT, name), mass_amu=amu, charge=ch, pos=pos, vel=vel,
+             fields=_fields_arr(f), lifetime=np.float64(lifetime), h0=np.float64(h0),
+             rtol=np.float64(rtol), atol=np.float64(atol), adaptive=np.int32(adaptive),
+             nsteps=np.int32(nsteps), pos_out=p, vel_out=v,
+             scalars=np.array([sc["lifetime"], sc["h"], sc["hnext"], sc["errold"], sc["reject"],
+                               sc["countstep"], sc["err"]]),
+             hist=hist)
+    print(name, n, nsteps, sc)
+
+
+def main():
+    if not oracle.ref_available():
+        raise SystemExit("oracle/_ref/ref_harness is not built (make -C oracle ref)")
+    # force(): Paul-trap RF+DC at t != 0, TOF kick, Coulomb; mixed species and charges
+    f = oracle.fields(trap=1, Vrf=50.0, Vdc=5.0, kappa=0.001, r0=0.04, z0=0.04, freq_rf=50e5,
+                      tof=1, E_kick=3.0, coulomb=1, coulomb_scale=1.0)
+    force_case("force_n500_trap_tof_coulomb.npz", 11, 500, 5e-4, [133.0, 85.0], [1, 1, 2, -1],
+               f, 1.234e-6)
+    # force(): Coulomb only, scaled, ragged size
+    f = oracle.fields(trap=0, coulomb=1, coulomb_scale=0.5)
+    force_case("force_n257_coulomb_scaled.npz", 12, 257, 1e-3, [39.0, 133.0], [1], f, 0.0)
+    # 200 adaptive steps at the reference tolerances (rtol = atol = 1e-7)
+    f = oracle.fields(trap=1, Vrf=0.0, Vdc=5.0, kappa=0.001, r0=0.04, z0=0.04, coulomb=1)
+    steps_case("steps_adaptive_n64.npz", 13, 64, 5e-4, f, 1e-9, 1e-7, 1e-7, True, 200)
+    # 20 fixed steps with a time-dependent RF field frozen at t_n in every stage
+    f = oracle.fields(trap=1, Vrf=50.0, Vdc=5.0, kappa=0.001, r0=0.04, z0=0.04, freq_rf=50e5,
+                      coulomb=1)
+    steps_case("steps_fixed_rf_n64.npz", 14, 64, 5e-4, f, 2e-9, 1e-7, 1e-7, False, 20,
+               lifetime=3.0e-7)
+
+
+if __name__ == "__main__":
+    main()

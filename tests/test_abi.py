@@ -1,0 +1,85 @@
+"""CPU-side checks of the product library: it builds, loads, and exports every
+symbol include/simbuca_hip.h declares; host-side helpers agree with the
+oracle.  No GPU work is launched here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+import simbuca_amd as S
+from simbuca_amd import synth
+from conftest import ROOT
+
+
+def _declared_symbols():
+    src = open(os.path.join(ROOT, "include", "simbuca_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"\b((?:cunbody1|sbk)_\w+)\s*\(", src)
+    return sorted(set(names))
+
+
+def test_header_and_binding_agree():
+    assert _declared_symbols() == sorted(S.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(S.LIB_PATH)
+    for name in _declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_struct_sizes_match_header():
+    # sbk_config: 5 ints, padding to 8, 3 doubles (x86-64 ABI)
+    assert ctypes.sizeof(S.Config) == 5 * 4 + 4 + 3 * 8
+    assert ctypes.sizeof(S.Fields) == ctypes.sizeof(O.Fields)
+    for (a, ta), (b, tb) in zip(S.Fields._fields_, O.Fields._fields_):
+        assert a == b and ta == tb
+
+
+def test_defaults_are_the_reference_defaults():
+    f = S.default_fields()
+    # bin/trapparameters.cpp:2-10, bin/ode.cpp:646-653
+    assert (f.trap, f.kappa, f.Vdc, f.Vrf, f.r0, f.z0, f.freq_rf) == (1, 0.001, 5, 50, 0.04, 0.04, 50e5)
+    assert (f.coulomb, f.coulomb_scale, f.tof) == (0, 1.0, 0)
+    cfg = S.Config()
+    S.lib().sbk_default_config(ctypes.byref(cfg))
+    assert (cfg.rtol, cfg.atol, cfg.h0, cfg.adaptive) == (1e-7, 1e-7, 1e-9, 1)
+
+
+def test_engine_fails_loudly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(S.SimbucaError):
+        S.Engine()
+
+
+def test_synth_masses_match_reference_rule():
+    amu = np.array([133.0, 85.0, 39.0, 1.00727638])
+    assert np.array_equal(synth.mass_kg(amu), O.mass_kg(amu))
+
+
+def test_synth_penning_frequencies():
+    nu_c, nu_p, nu_m, nu_z = synth.penning_frequencies(133.0)
+    assert nu_c == pytest.approx(808216.5, rel=1e-6)
+    assert nu_m == pytest.approx(940.62, rel=1e-4)
+    assert nu_p + nu_m == pytest.approx(nu_c, rel=1e-12)
+    assert 2 * nu_p * nu_m == pytest.approx(nu_z ** 2, rel=1e-9)
+
+
+def test_synth_workloads_shapes():
+    w = synth.workload(3, n_override=2048)
+    c = w.clouds[0]
+    assert c.pos.shape == (2048, 3) and c.charge.dtype == np.int32
+    assert c.pos.std() == pytest.approx(1.5e-3, rel=0.05)
+    sig_v = np.sqrt(1.3806504e-23 * 300.0 / c.mass[0])
+    assert c.vel.std() == pytest.approx(sig_v, rel=0.05)
+    w4 = synth.workload(4, n_override=300)
+    sizes = [cl.n for cl in w4.clouds]
+    assert len(sizes) == 256 and min(sizes) >= 64 and max(sizes) <= 300
+    w2 = synth.workload(2, n_override=4000)
+    frac = np.mean(w2.clouds[0].mass_amu == 85.0)
+    assert 0.45 < frac < 0.55

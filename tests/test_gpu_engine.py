@@ -1,0 +1,301 @@
+"""GPU parity of the HIP engine against the CPU oracle and the reference's
+golden vectors.  Every call goes through the C ABI of libsimbuca_hip.so.
+
+Tolerances:
+* FP64-exact Coulomb, force() without time-dependent cosines, stage
+  combinations: bit-identical (same operations in the same order, IEEE
+  sqrt/div on both sides, no FMA contraction).
+* Terms with cos(): relative 1e-14 (device and host libm may differ by an ulp).
+* Trajectories: the device sums the error norm in a different association
+  than the serial CPU loop, so h differs in the last bits: relative 1e-9 on
+  states after 200 steps, exact accept/reject sequence.
+* Mixed precision Coulomb: normwise ||a - a64|| / ||a64|| <= 1e-6.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import simbuca_amd as S
+from simbuca_amd import synth
+from conftest import golden_fields, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _both_fields(**kw):
+    return S.default_fields(**kw), O.fields(**kw)
+
+
+def _rand_cloud(rng, n, sigma=5e-4, species=(133.0, 85.0), charges=(1, 1, 2, -1)):
+    amu = rng.choice(np.array(species), n)
+    ch = rng.choice(np.array(charges, dtype=np.int32), n).astype(np.int32)
+    pos = rng.normal(0, sigma, (n, 3))
+    vel = rng.normal(0, 137.0, (n, 3))
+    return amu, O.mass_kg(amu), ch, pos, vel
+
+
+def _rows_error(a, ref):
+    return np.linalg.norm(a - ref) / np.linalg.norm(ref)
+
+
+# ---------------------------------------------------------------------------
+# force() and the Coulomb sum
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n", [1, 2, 255, 256, 257, 1000])
+def test_coulomb_fp64_exact_is_bit_identical(rng, n):
+    amu, m, ch, pos, vel = _rand_cloud(rng, n)
+    sf, of = _both_fields(trap=0, coulomb=1, coulomb_scale=1.0)
+    with S.Engine(precision="fp64_exact") as e:
+        e.add_cloud(pos, vel, m, ch, sf)
+        f = e.eval_coulomb(0, "fp64_exact")
+    ref = O.coulomb_rows(pos, m, ch, 1.0, np.arange(n, dtype=np.int32))
+    assert np.array_equal(f, ref)
+
+
+@pytest.mark.parametrize("name", ["force_n500_trap_tof_coulomb.npz", "force_n257_coulomb_scaled.npz"])
+def test_force_matches_reference_golden(name):
+    g = load_golden(name)
+    sf = golden_fields(g["fields"], S.default_fields)
+    m = O.mass_kg(g["mass_amu"])
+    with S.Engine(precision="fp64_exact") as e:
+        e.add_cloud(g["pos"], g["vel"], m, g["charge"], sf, lifetime=float(g["lifetime"]))
+        out = e.eval_force(0)
+    ref = g["derivs"]
+    if sf.trap and sf.Vrf != 0.0:
+        # only cos(omega t) may differ by an ulp
+        assert np.allclose(out, ref, rtol=1e-14, atol=1e-14 * np.abs(ref).max())
+    else:
+        assert np.array_equal(out, ref)
+
+
+def test_force_bit_identical_without_cosines(rng):
+    amu, m, ch, pos, vel = _rand_cloud(rng, 700)
+    kw = dict(trap=1, Vrf=0.0, Vdc=4.0, kappa=0.3, r0=0.01, z0=0.02, tof=1, E_kick=-2.0,
+              coulomb=1, coulomb_scale=0.8, V0=10.0, d0=11e-3, Bz=7.0, delta=250.0)
+    sf, of = _both_fields(**kw)
+    with S.Engine(precision="fp64_exact") as e:
+        e.add_cloud(pos, vel, m, ch, sf, lifetime=2.5e-6)
+        out = e.eval_force(0)
+    ref = O.force(pos, vel, m, ch, of, 2.5e-6)
+    assert np.array_equal(out, ref)
+
+
+def test_force_time_dependent_terms(rng):
+    amu, m, ch, pos, vel = _rand_cloud(rng, 300)
+    kw = dict(trap=1, Vrf=40.0, Vdc=4.0, kappa=0.3, r0=0.01, z0=0.02, freq_rf=1.3e6, coulomb=1,
+              Ed=12.0, fd=9.4e2, phid=0.3, Uq=0.2, rq=9e-3, fq=8.08e5, phiq=-0.7)
+    sf, of = _both_fields(**kw)
+    with S.Engine(precision="fp64_exact") as e:
+        e.add_cloud(pos, vel, m, ch, sf, lifetime=3.7e-6)
+        out = e.eval_force(0)
+    ref = O.force(pos, vel, m, ch, of, 3.7e-6)
+    assert np.allclose(out, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6), ("mixed_kahan", 1e-6)])
+def test_coulomb_precisions_at_65536(precision, tol):
+    w = synth.workload(3)
+    c = w.clouds[0]
+    rows = np.random.Generator(np.random.PCG64(7)).choice(c.n, 192, replace=False).astype(np.int32)
+    ref = O.coulomb_rows(c.pos, c.mass, c.charge, 1.0, rows)
+    with S.Engine(precision=precision) as e:
+        e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
+        f = e.eval_coulomb(0, precision)
+        exact = e.eval_coulomb(0, "fp64_exact")
+    assert np.array_equal(exact[rows], ref)
+    err = _rows_error(f, exact)
+    assert err <= tol, err
+    # Newton's third law holds to the same accuracy (equal masses and charges)
+    assert np.linalg.norm(f.sum(0)) <= 10 * tol * np.abs(f).sum(0).max()
+
+
+@pytest.mark.parametrize("tpt", [1, 2, 4])
+@pytest.mark.parametrize("slices", [0, 1, 3, 7])
+def test_launch_shapes_agree(rng, tpt, slices):
+    amu, m, ch, pos, vel = _rand_cloud(rng, 3001, sigma=1e-3)
+    sf, _ = _both_fields(trap=0, coulomb=1)
+    with S.Engine(precision="mixed") as e:
+        e.add_cloud(pos, vel, m, ch, sf)
+        e.set_launch(tpt, slices)
+        mixed = e.eval_coulomb(0, "mixed")
+        fp64 = e.eval_coulomb(0, "fp64")
+        exact = e.eval_coulomb(0, "fp64_exact")
+    assert _rows_error(fp64, exact) < 1e-13
+    assert _rows_error(mixed, exact) < 1e-6
+
+
+def test_multi_cloud_equals_single_clouds(rng):
+    sizes = [64, 1, 300, 0, 1029]
+    clouds = [_rand_cloud(rng, n) for n in sizes]
+    sf, _ = _both_fields(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0)
+    with S.Engine(precision="fp64_exact") as e:
+        for _, m, ch, pos, vel in clouds:
+            e.add_cloud(pos, vel, m, ch, sf)
+        outs = [e.eval_force(i) for i in range(len(sizes))]
+    for (amu, m, ch, pos, vel), out in zip(clouds, outs):
+        if len(m) == 0:
+            assert out.shape == (0, 3)
+            continue
+        ref = O.force(pos, vel, m, ch, O.fields(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0), 0.0)
+        assert np.array_equal(out, ref)
+
+
+# ---------------------------------------------------------------------------
+# Dormand-Prince steps
+# ---------------------------------------------------------------------------
+
+def _golden_engine(g, use_graph):
+    sf = golden_fields(g["fields"], S.default_fields)
+    m = O.mass_kg(g["mass_amu"])
+    e = S.Engine(precision="fp64_exact", adaptive=bool(g["adaptive"]), use_graph=use_graph,
+                 rtol=float(g["rtol"]), atol=float(g["atol"]), h0=float(g["h0"]),
+                 time_mode=S.TIME_FROZEN)
+    e.add_cloud(g["pos"], g["vel"], m, g["charge"], sf, lifetime=float(g["lifetime"]))
+    return e
+
+
+def test_fixed_steps_match_reference_golden():
+    g = load_golden("steps_fixed_rf_n64.npz")
+    with _golden_engine(g, use_graph=False) as e:
+        for _ in range(int(g["nsteps"])):
+            e.step()
+        pos, vel = e.state(0)
+        st = e.status(0)
+    assert st["lifetime"] == g["scalars"][0]
+    # only the RF cosine may differ by an ulp
+    assert np.allclose(pos, g["pos_out"], rtol=1e-13, atol=1e-13 * np.abs(g["pos_out"]).max())
+    assert np.allclose(vel, g["vel_out"], rtol=1e-12, atol=1e-12 * np.abs(g["vel_out"]).max())
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_adaptive_steps_match_reference_golden(use_graph):
+    g = load_golden("steps_adaptive_n64.npz")
+    with _golden_engine(g, use_graph=use_graph) as e:
+        hist = []
+        for _ in range(int(g["nsteps"])):
+            hist.append(e.status(0)["h"])
+            e.step()
+        pos, vel = e.state(0)
+        st = e.status(0)
+    assert st["attempts"] == g["scalars"][5]          # same accept/reject sequence
+    assert np.allclose(np.array(hist), g["hist"][:, 0], rtol=1e-9, atol=0)
+    assert st["lifetime"] == pytest.approx(g["scalars"][0], rel=1e-9)
+    assert np.allclose(pos, g["pos_out"], rtol=1e-8, atol=1e-8 * np.abs(g["pos_out"]).max())
+    assert np.allclose(vel, g["vel_out"], rtol=1e-8, atol=1e-8 * np.abs(g["vel_out"]).max())
+
+
+def test_single_attempt_stage_combination_bit_identical(rng):
+    # one fixed step without cosines: every stage combination bit-identical
+    amu, m, ch, pos, vel = _rand_cloud(rng, 333)
+    kw = dict(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0, delta=300.0)
+    sf, of = _both_fields(**kw)
+    with S.Engine(precision="fp64_exact", adaptive=False, use_graph=False, h0=3e-9) as e:
+        e.add_cloud(pos, vel, m, ch, sf)
+        e.step()
+        gp, gv = e.state(0)
+    c = O.Cloud(pos, vel, m, ch, of, h0=3e-9, adaptive=False, time_mode=O.TIME_FROZEN)
+    c.step()
+    op, ov, _ = c.state()
+    assert np.array_equal(gp, op)
+    assert np.array_equal(gv, ov)
+
+
+def test_graph_and_eager_agree(rng):
+    amu, m, ch, pos, vel = _rand_cloud(rng, 900, sigma=1e-3)
+    sf, _ = _both_fields(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0, Ed=10.0, fd=940.0)
+    res = []
+    for use_graph in (False, True):
+        with S.Engine(precision="mixed", use_graph=use_graph, rtol=1e-9, atol=1e-9,
+                      time_mode=S.TIME_STAGE) as e:
+            e.add_cloud(pos, vel, m, ch, sf)
+            e.advance(2e-7)
+            res.append((*e.state(0), e.status(0)))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
+    assert res[0][2]["attempts"] == res[1][2]["attempts"]
+
+
+def test_penning_physics_vs_oracle(rng):
+    # stage-time mode with B, Penning quadrupole, dipole + quadrupole drive, friction
+    amu, m, ch, pos, vel = _rand_cloud(rng, 200, sigma=5e-4, species=(133.0,), charges=(1,))
+    nu_c, _, nu_m, _ = synth.penning_frequencies(133.0)
+    kw = dict(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0, Ed=10.0, fd=nu_m, Uq=0.1,
+              rq=11e-3, fq=nu_c, delta=1e3)
+    sf, of = _both_fields(**kw)
+    with S.Engine(precision="fp64_exact", rtol=1e-9, atol=1e-9, time_mode=S.TIME_STAGE) as e:
+        e.add_cloud(pos, vel, m, ch, sf)
+        for _ in range(40):
+            e.step()
+        gp, gv = e.state(0)
+        st = e.status(0)
+    c = O.Cloud(pos, vel, m, ch, of, h0=1e-9, rtol=1e-9, atol=1e-9, time_mode=O.TIME_STAGE)
+    for _ in range(40):
+        c.step()
+    op, ov, sc = c.state()
+    assert st["attempts"] == sc["attempts"]
+    assert st["lifetime"] == pytest.approx(sc["lifetime"], rel=1e-10)
+    assert np.allclose(gp, op, rtol=1e-9, atol=1e-9 * np.abs(op).max())
+    assert np.allclose(gv, ov, rtol=1e-9, atol=1e-9 * np.abs(ov).max())
+
+
+def test_mixed_short_horizon_deviation(rng):
+    # 100 fixed steps: mixed vs FP64 engine trajectories stay close
+    w = synth.workload(2, n_override=4096)
+    c = w.clouds[0]
+    out = {}
+    for p in ("fp64", "mixed"):
+        with S.Engine(precision=p, adaptive=False, h0=2e-9, time_mode=S.TIME_STAGE) as e:
+            e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
+            e.attempts(100)
+            e.synchronize()
+            out[p] = e.state(0)
+    dev = np.abs(out["mixed"][0] - out["fp64"][0]).max()
+    assert dev < 1e-9 * np.abs(out["fp64"][0]).max()
+
+
+def test_advance_reaches_t_end_and_counts(rng):
+    amu, m, ch, pos, vel = _rand_cloud(rng, 128, species=(133.0,), charges=(1,))
+    sf, _ = _both_fields(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0)
+    with S.Engine(precision="mixed", rtol=1e-8, atol=1e-8, time_mode=S.TIME_STAGE) as e:
+        e.add_cloud(pos, vel, m, ch, sf, lifetime=1e-6)
+        e.reset_stats()
+        e.advance(3e-7)
+        st = e.status(0)
+        stats = e.stats()
+    assert st["lifetime"] >= 1.3e-6 and st["active"] == 0
+    assert stats["steps"] == st["steps"]
+    assert stats["pair_interactions"] == st["attempts"] * 7 * 128 * 127
+
+
+def test_underflow_is_reported():
+    sf, _ = _both_fields(trap=1, Vrf=0.0, kappa=1.0, Vdc=5.0, z0=11e-3, r0=1.0)
+    with S.Engine(precision="fp64_exact", rtol=0.0, atol=1e-300, h0=1e-6) as e:
+        e.add_cloud([[1e-3, 0, 1e-3]], [[100.0, 0, 0]], O.mass_kg([133.0]), [1], sf)
+        with pytest.raises(S.StepsizeUnderflow):
+            e.step()
+
+
+# ---------------------------------------------------------------------------
+# cunbody1_force ABI
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n", [1, 100, 1000, 4099])
+def test_cunbody_abi_exact(rng, n):
+    x = rng.normal(0, 1e-3, (n, 3))
+    mj = np.ones(n)
+    eps2 = float(np.float32(1e-23))   # bin/cunbody.cpp:11,66
+    ai = S.cunbody1_force(x, mj, x.copy(), eps2, precision="fp64_exact")
+    ref = O.cunbody_force(x, mj, x, eps2)
+    assert np.array_equal(ai, ref)
+
+
+def test_cunbody_abi_fp64_and_mixed(rng):
+    ni, nj = 3000, 5000
+    xj = rng.normal(0, 1e-3, (nj, 3)); xi = rng.normal(0, 1e-3, (ni, 3))
+    mj = rng.uniform(0.5, 2.0, nj)
+    ref = O.cunbody_force(xj, mj, xi, 1e-20)
+    a64 = S.cunbody1_force(xj, mj, xi, 1e-20, precision="fp64")
+    amx = S.cunbody1_force(xj, mj, xi, 1e-20, precision="mixed")
+    assert _rows_error(a64, ref) < 1e-13
+    assert _rows_error(amx, ref) < 1e-6
