@@ -45,13 +45,19 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_engine(S, w, precision, steps, warmup, timing, use_graph=True, tune=None):
-    c = w.clouds[0]
+def make_engine(S, w, precision, timing, use_graph=True, tune=None):
     e = S.Engine(precision=precision, time_mode=w.time_mode, rtol=w.rtol, atol=w.atol, h0=w.h0,
                  use_graph=use_graph, timing=timing)
-    e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
+    for c in w.clouds:
+        e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
     if tune is not None:
         e.set_launch(*tune)
+    return e
+
+
+def run_engine(S, w, precision, steps, warmup, timing, use_graph=True, tune=None):
+    """`warmup` then `steps` timed attempts of every cloud (inputs resident)."""
+    e = make_engine(S, w, precision, timing, use_graph, tune)
     e.attempts(warmup)                  # uploads, activates with no end time, warms up
     e.synchronize()
     e.reset_stats()
@@ -64,29 +70,49 @@ def run_engine(S, w, precision, steps, warmup, timing, use_graph=True, tune=None
     return e, dt, st, status
 
 
+def run_to_end(S, w, precision, tune=None):
+    """The whole simulated interval of the workload (move_particles to t_end)."""
+    e = make_engine(S, w, precision, timing=False, tune=tune)
+    e.attempts(0)
+    e.synchronize()
+    e.reset_stats()
+    t0 = time.perf_counter()
+    e.advance(w.t_sim)
+    dt = time.perf_counter() - t0
+    st = e.stats()
+    e.close()
+    return dt, st
+
+
 def force_error(S, w, e_mixed):
+    """Mixed Coulomb sum against the FP64-exact sum, both on the GPU at the
+    engine's current positions; the FP64-exact rows are checked against the
+    CPU oracle on a sample."""
+    import oracle
     c = w.clouds[0]
+    pos, _ = e_mixed.state(0)
     a64 = e_mixed.eval_coulomb(0, "fp64_exact")
     amx = e_mixed.eval_coulomb(0, "mixed")
     d = np.linalg.norm(amx - a64, axis=1)
-    a = np.linalg.norm(a64, axis=1)
-    rel = d / a
-    # condition-weighted: |da_i| / sum_j |a_ij| with sum_j |a_ij| = k_i sum_j q_j / r_ij^2
-    return a64, {
+    rel = d / np.linalg.norm(a64, axis=1)
+    rows = np.random.Generator(np.random.PCG64(11)).choice(c.n, 64, replace=False).astype(np.int32)
+    ref = oracle.coulomb_rows(pos, c.mass, c.charge, 1.0, rows)
+    # condition-weighted error |da_i| / sum_j |a_ij| on the sampled rows
+    kc = 8.987551787e9 * 1.60217733e-19 ** 2 * c.charge[rows] / c.mass[rows]
+    cond = np.empty(rows.size)
+    for k, i in enumerate(rows):
+        r2 = ((pos - pos[i]) ** 2).sum(1)
+        r2[i] = np.inf
+        cond[k] = d[i] / (abs(kc[k]) * (np.abs(c.charge) / r2).sum())
+    return {
         "normwise": float(np.linalg.norm(amx - a64) / np.linalg.norm(a64)),
         "per_ion_p50": float(np.percentile(rel, 50)),
         "per_ion_p99": float(np.percentile(rel, 99)),
         "per_ion_max": float(rel.max()),
-        "reference": "FP64-exact GPU sum (bit-identical to the CPU oracle on sampled rows)",
+        "cond_weighted_max_sampled": float(cond.max()),
+        "reference": "FP64-exact GPU sum over all ions",
+        "fp64_exact_rows_equal_oracle": bool(np.array_equal(a64[rows], ref)),
     }
-
-
-def oracle_rows_check(w, a64):
-    import oracle
-    c = w.clouds[0]
-    rows = np.random.Generator(np.random.PCG64(11)).choice(c.n, 64, replace=False).astype(np.int32)
-    ref = oracle.coulomb_rows(c.pos, c.mass, c.charge, 1.0, rows)
-    return bool(np.array_equal(a64[rows], ref))
 
 
 def cpu_baseline(w):
@@ -120,6 +146,65 @@ def cpu_baseline(w):
     }
 
 
+def _device_copy(nbytes, reps):
+    """hipMemcpy device-to-device of `nbytes`, `reps` times (a blit kernel that
+    reads and writes exactly nbytes: the calibration for FETCH_SIZE/WRITE_SIZE)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
+def kernels_only(S, w):
+    """For counter runs: 5 mixed and 2 FP64 Coulomb launches at the default
+    launch shape, then 3 device copies of 1 GiB whose traffic is known."""
+    c = w.clouds[0]
+    e = S.Engine(precision="mixed")
+    e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
+    log(f"mixed coulomb {e.time_coulomb('mixed', 5):.4f} ms/launch")
+    log(f"fp64 coulomb {e.time_coulomb('fp64', 2):.4f} ms/launch")
+    e.close()
+    _device_copy(1 << 30, 3)
+    log("copied 3 x 1 GiB")
+
+
+def other_workload(S, w, args, precision):
+    """Secondary workloads: attempt throughput of the mixed and FP64 engines
+    (and optionally the whole simulated interval); one JSON line."""
+    sizes = [cl.n for cl in w.clouds]
+    tune = None
+    if args.tune:
+        e = make_engine(S, w, precision, timing=False)
+        tune = e.autotune(precision)
+        e.close()
+    res = {"workload": w.name, "clouds": len(sizes), "ions": int(sum(sizes)),
+           "pairs_per_attempt": 7.0 * sum(float(n) * (n - 1) for n in sizes), "tune": tune}
+    for p in (precision, "fp64"):
+        e, dt, st, _ = run_engine(S, w, p, args.steps, args.warmup, timing=True, tune=tune)
+        res[p] = {"attempts": st["attempts"], "steps": st["steps"], "seconds": dt,
+                  "ms_per_attempt": 1e3 * dt / args.steps,
+                  "pair_interactions_per_s": st["pair_interactions"] / dt,
+                  "ion_steps_per_s": st["ion_steps"] / dt,
+                  "coulomb_ms_per_launch": st["coulomb_ms"] / max(1, st["coulomb_launches"])}
+        e.close()
+        log(f"{p}: {res[p]}")
+        if args.to_end:
+            dte, ste = run_to_end(S, w, p, tune)
+            res[p]["to_end"] = {"t_sim": w.t_sim, "seconds": dte, "steps": ste["steps"],
+                                "attempts": ste["attempts"],
+                                "ion_steps_per_s": ste["ion_steps"] / dte,
+                                "pair_interactions_per_s": ste["pair_interactions"] / dte}
+            log(f"{p} to end: {res[p]['to_end']}")
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +214,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", action="store_true", help="autotune the Coulomb launch shape first")
     ap.add_argument("--n", type=int, default=0, help="override the ion count (testing)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only launch the Coulomb kernels and a 1 GiB device copy (profiler runs)")
+    ap.add_argument("--workload", type=int, default=3,
+                    help="1: 500 ions, 2: 10k two species, 3: 65,536 ions (default), "
+                         "4: 256-cloud ensemble, 5: 262,144 ions")
+    ap.add_argument("--to-end", action="store_true",
+                    help="also run the workload's whole simulated interval (mixed and fp64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the engine does not "
@@ -138,11 +230,20 @@ def main():
     import simbuca_amd as S
     from simbuca_amd import synth
 
-    w = synth.workload(3, n_override=args.n or None)
+    w = synth.workload(args.workload, n_override=args.n or None)
     c = w.clouds[0]
     n = c.n
+    sizes = [cl.n for cl in w.clouds]
+    precision = "mixed_kahan" if w.precision == "mixed_kahan" else "mixed"
     info = S.device_info()
     log(f"device {info}; workload {w.name}; n={n}")
+
+    if args.kernels_only:
+        kernels_only(S, w)
+        return
+    if args.workload != 3:
+        other_workload(S, w, args, precision)
+        return
 
     tune = None
     if args.tune:
@@ -160,8 +261,7 @@ def main():
     coul_ms = st["coulomb_ms"] / max(1, st["coulomb_launches"])
     log(f"mixed: {args.steps} attempts in {dt:.3f}s, {st['steps']} accepted; "
         f"{value:.4e} pairs/s, {ion_steps:.4e} ion-steps/s, Coulomb {coul_ms:.4f} ms/launch")
-    a64, ferr = force_error(S, w, e)
-    ferr["oracle_rows_bit_identical"] = oracle_rows_check(w, a64)
+    ferr = force_error(S, w, e)
     log(f"force error {ferr}")
     e.close()
 

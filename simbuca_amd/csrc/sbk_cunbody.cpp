@@ -59,14 +59,14 @@ void ensure(int ni, int nj)
     if (dev != g.device) {
         if (g.stream) {
             // buffers belong to the previous device
-            hipSetDevice(g.device);
-            hipStreamSynchronize(g.stream);
-            hipFree(g.d_buf);
-            hipFree(g.d_mf);
-            hipFree(g.d_items);
-            hipHostFree(g.h_buf);
-            hipStreamDestroy(g.stream);
-            hipSetDevice(dev);
+            (void)hipSetDevice(g.device);
+            (void)hipStreamSynchronize(g.stream);
+            (void)hipFree(g.d_buf);
+            (void)hipFree(g.d_mf);
+            (void)hipFree(g.d_items);
+            (void)hipHostFree(g.h_buf);
+            (void)hipStreamDestroy(g.stream);
+            (void)hipSetDevice(dev);
         }
         g = CunbodyState{g.device, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, 0, 0, 0,
                          0, 0, g.precision, 256};
@@ -78,9 +78,9 @@ void ensure(int ni, int nj)
     }
     if (ni > g.cap_i || nj > g.cap_j) {
         CHK(hipStreamSynchronize(g.stream));
-        hipFree(g.d_buf);
-        hipFree(g.d_mf);
-        if (g.h_buf) hipHostFree(g.h_buf);
+        (void)hipFree(g.d_buf);
+        (void)hipFree(g.d_mf);
+        if (g.h_buf) (void)hipHostFree(g.h_buf);
         g.cap_i = std::max(ni, g.cap_i);
         g.cap_j = std::max(nj, g.cap_j);
         const size_t doubles = 6 * (size_t)g.cap_i + 4 * (size_t)g.cap_j;
@@ -115,7 +115,7 @@ void build_items(int ni, int nj, int tpt)
             items.push_back(w);
         }
     if ((int)items.size() > g.cap_items) {
-        hipFree(g.d_items);
+        (void)hipFree(g.d_items);
         g.cap_items = (int)items.size();
         CHK(hipMalloc(&g.d_items, sizeof(sbk::CoulombItem) * g.cap_items));
     }
@@ -187,7 +187,7 @@ void cunbody1_force_begin(double xj[][3], double mj[], double xi[][3], double ep
     const int S = g.nitems / ((ni + blk - 1) / blk);   // items = slices x target blocks
     const long long stride = ci;
     if ((long long)S * 3 * stride > partials_cap) {
-        hipFree(partials);
+        (void)hipFree(partials);
         partials_cap = (long long)S * 3 * stride;
         CHK(hipMalloc(&partials, sizeof(double) * partials_cap));
     }
@@ -206,6 +206,8 @@ void cunbody1_force_begin(double xj[][3], double mj[], double xi[][3], double ep
     a.items = g.d_items;
     a.nitems = g.nitems;
     a.self_skip = 0;
+    a.unit_weights = 1;
+    for (int j = 0; j < nj && a.unit_weights; ++j) a.unit_weights = mj[j] == 1.0;
     a.state = nullptr;
     a.eps_d = eps2;
     a.eps_f = std::max((float)eps2, 1e-23f);   // FP32 needs eps >= 1e-23 to keep r=0 finite

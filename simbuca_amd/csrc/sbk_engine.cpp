@@ -83,13 +83,16 @@ struct sbk_engine {
     // device memory
     double *d_f64 = nullptr;   // all FP64 SoA arrays
     float *d_qf = nullptr;
-    int *d_cloud_of = nullptr;
+    sbk::IonBlock *d_blocks = nullptr;
+    double *d_errb = nullptr;
+    std::vector<int> blk0, nblk;
     double *d_cp = nullptr;
     int cp_slices = 0;
     sbk::CloudParams *d_params = nullptr;
     sbk::CloudState *d_state = nullptr;
     sbk::CloudState *h_state = nullptr;   // pinned mirror
     double *d_kc = nullptr, *d_qd = nullptr;
+    bool unit_charges = false;   // every ion has charge +1
     sbk::EngineDev dev{};
 
     // launch shape
@@ -151,8 +154,8 @@ int kp(int precision)
 
 void destroy_graph(sbk_engine *e)
 {
-    if (e->graph_exec) hipGraphExecDestroy(e->graph_exec);
-    if (e->graph) hipGraphDestroy(e->graph);
+    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
     e->graph_exec = nullptr;
     e->graph = nullptr;
 }
@@ -160,8 +163,8 @@ void destroy_graph(sbk_engine *e)
 void free_lists(sbk_engine *e)
 {
     for (auto &kv : e->lists) {
-        hipFree(kv.second.d_items);
-        hipFree(kv.second.d_slices);
+        (void)hipFree(kv.second.d_items);
+        (void)hipFree(kv.second.d_slices);
     }
     e->lists.clear();
     destroy_graph(e);
@@ -183,7 +186,11 @@ int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
     long long total_blocks = 0;
     for (auto &c : e->clouds)
         if (c.n > 0 && (!only_coulomb || c.f.coulomb)) total_blocks += (c.n + blk - 1) / blk;
-    const long long target = 8LL * e->cus;
+    // automatic split: two full rounds of resident work-groups, so that no
+    // partially filled last round (tail) is left on the CUs
+    const long long capacity =
+        (long long)sbk::coulomb_blocks_per_cu(precision, tpt, e->unit_charges ? 1 : 0) * e->cus;
+    const long long target = 2 * capacity;
     ItemList L;
     std::vector<sbk::CoulombItem> items;
     L.slices.assign(e->clouds.size(), 1);
@@ -196,8 +203,8 @@ int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
             if (sreq > 0)
                 S = sreq;
             else {
-                long long want = total_blocks > 0 ? (target + total_blocks - 1) / total_blocks : 1;
-                long long cap = std::max(1, c.n / 1024);
+                long long want = total_blocks > 0 ? (target + total_blocks / 2) / total_blocks : 1;
+                long long cap = std::max(1, c.n / 512);
                 S = (int)std::max(1LL, std::min(want, cap));
             }
         }
@@ -244,10 +251,19 @@ int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
     return 0;
 }
 
+sbk::CloudParams cloud_params(const sbk_engine *e, size_t ci)
+{
+    const HostCloud &c = e->clouds[ci];
+    sbk::CloudParams p = make_params(c.f, c.off, c.n);
+    p.blk0 = e->blk0[ci];
+    p.nblk = e->nblk[ci];
+    return p;
+}
+
 int upload_params(sbk_engine *e)
 {
     std::vector<sbk::CloudParams> ps;
-    for (auto &c : e->clouds) ps.push_back(make_params(c.f, c.off, c.n));
+    for (size_t ci = 0; ci < e->clouds.size(); ++ci) ps.push_back(cloud_params(e, ci));
     HIP_TRY(hipMemcpyAsync(e->d_params, ps.data(), sizeof(sbk::CloudParams) * ps.size(),
                            hipMemcpyHostToDevice, e->stream));
     return 0;
@@ -268,7 +284,7 @@ void ion_constants(const HostCloud &c, double *qm, double *kc, double *qd, float
 // index of the FP64 SoA arrays inside d_f64
 enum {
     A_P0 = 0, A_V0 = 3, A_PS = 6, A_KV = 9 /* KV2..KV7: 18 */, A_KA = 27 /* KA1..KA7: 21 */,
-    A_ERR = 48, A_DER = 49 /* 3 */, A_QM = 52, A_KC = 53, A_QD = 54, A_COUNT = 55
+    A_DER = 48 /* 3 */, A_QM = 51, A_KC = 52, A_QD = 53, A_COUNT = 54
 };
 
 int finalize(sbk_engine *e)
@@ -285,7 +301,23 @@ int finalize(sbk_engine *e)
     HIP_TRY(hipMalloc(&e->d_f64, sizeof(double) * S * A_COUNT));
     HIP_TRY(hipMemsetAsync(e->d_f64, 0, sizeof(double) * S * A_COUNT, e->stream));
     HIP_TRY(hipMalloc(&e->d_qf, sizeof(float) * S));
-    HIP_TRY(hipMalloc(&e->d_cloud_of, sizeof(int) * S));
+    // ion-block table: at most 256 ions of one cloud per block
+    std::vector<sbk::IonBlock> blocks;
+    e->blk0.assign(e->clouds.size(), 0);
+    e->nblk.assign(e->clouds.size(), 0);
+    for (size_t ci = 0; ci < e->clouds.size(); ++ci) {
+        const HostCloud &c = e->clouds[ci];
+        e->blk0[ci] = (int)blocks.size();
+        for (int b = 0; b < c.n; b += 256)
+            blocks.push_back(sbk::IonBlock{(int)ci, c.off + b, c.off + std::min(c.n, b + 256), 0});
+        e->nblk[ci] = (int)blocks.size() - e->blk0[ci];
+    }
+    HIP_TRY(hipMalloc(&e->d_blocks, sizeof(sbk::IonBlock) * std::max<size_t>(1, blocks.size())));
+    HIP_TRY(hipMalloc(&e->d_errb, sizeof(double) * std::max<size_t>(1, blocks.size())));
+    if (!blocks.empty())
+        HIP_TRY(hipMemcpyAsync(e->d_blocks, blocks.data(), sizeof(sbk::IonBlock) * blocks.size(),
+                               hipMemcpyHostToDevice, e->stream));
+    e->dev.nblocks = (int)blocks.size();
     HIP_TRY(hipMalloc(&e->d_params, sizeof(sbk::CloudParams) * e->clouds.size()));
     HIP_TRY(hipMalloc(&e->d_state, sizeof(sbk::CloudState) * e->clouds.size()));
     HIP_TRY(hipHostMalloc(&e->h_state, sizeof(sbk::CloudState) * e->clouds.size(),
@@ -294,7 +326,6 @@ int finalize(sbk_engine *e)
     // host staging in SoA, then one copy per array
     std::vector<double> soa(S * 6, 0.0), cst(S * 3, 0.0);
     std::vector<float> qf(S, 0.f);
-    std::vector<int> cof(S, 0);
     for (size_t ci = 0; ci < e->clouds.size(); ++ci) {
         HostCloud &c = e->clouds[ci];
         for (int i = 0; i < c.n; ++i) {
@@ -302,7 +333,6 @@ int finalize(sbk_engine *e)
                 soa[k * S + c.off + i] = c.pos[3 * i + k];
                 soa[(3 + k) * S + c.off + i] = c.vel[3 * i + k];
             }
-            cof[c.off + i] = (int)ci;
         }
         ion_constants(c, &cst[c.off], &cst[S + c.off], &cst[2 * S + c.off], &qf[c.off]);
     }
@@ -313,8 +343,7 @@ int finalize(sbk_engine *e)
                            hipMemcpyHostToDevice, e->stream));
     HIP_TRY(hipMemcpyAsync(e->d_qf, qf.data(), sizeof(float) * S, hipMemcpyHostToDevice,
                            e->stream));
-    HIP_TRY(hipMemcpyAsync(e->d_cloud_of, cof.data(), sizeof(int) * S, hipMemcpyHostToDevice,
-                           e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));   // pageable staging goes out of scope
     std::vector<sbk::CloudState> st(e->clouds.size());
     for (size_t ci = 0; ci < e->clouds.size(); ++ci) {
         sbk::CloudState s{};
@@ -342,13 +371,13 @@ int finalize(sbk_engine *e)
         for (int s = 2; s <= 7; ++s) d.KV[s][k] = base + (A_KV + (s - 2) * 3 + k) * S;
         for (int s = 1; s <= 7; ++s) d.KA[s][k] = base + (A_KA + (s - 1) * 3 + k) * S;
     }
-    d.ERR = base + A_ERR * S;
+    d.ERRB = e->d_errb;
     d.derivs_out = base + A_DER * S;
     d.stride = S;
     d.qm = base + A_QM * S;
     e->d_kc = base + A_KC * S;
     e->d_qd = base + A_QD * S;
-    d.cloud_of = e->d_cloud_of;
+    d.blocks = e->d_blocks;
     d.params = e->d_params;
     d.state = e->d_state;
     d.ntot = e->ntot;
@@ -359,6 +388,9 @@ int finalize(sbk_engine *e)
     d.atol = e->cfg.atol;
     d.CP = nullptr;
     d.slices = nullptr;
+    e->unit_charges = true;
+    for (auto &c : e->clouds)
+        for (int q : c.charge) e->unit_charges = e->unit_charges && q == 1;
     e->attempts_at_reset.assign(e->clouds.size(), 0);
     e->steps_at_reset.assign(e->clouds.size(), 0);
     e->finalized = true;
@@ -383,6 +415,7 @@ sbk::CoulombArgs coulomb_args(sbk_engine *e, const ItemList &L, double *const *p
     a.items = L.d_items;
     a.nitems = L.nitems;
     a.self_skip = 1;
+    a.unit_weights = e->unit_charges ? 1 : 0;
     a.state = gated ? e->d_state : nullptr;
     a.eps_f = 1e-23f;   // (float) softening of bin/cunbody.cpp:11; keeps the self pair finite
     a.eps_d = 0.0;      // FP64-exact: no softening (bin/force.cpp:163)
@@ -459,7 +492,7 @@ int run_attempts(sbk_engine *e, int count)
             hipGraph_t g = nullptr;
             hipError_t ce = hipStreamEndCapture(e->stream, &g);
             if (rc) {
-                if (g) hipGraphDestroy(g);
+                if (g) (void)hipGraphDestroy(g);
                 return rc;
             }
             if (ce != hipSuccess) return fail(SBK_EHIP, "graph capture failed: %s", hipGetErrorString(ce));
@@ -569,21 +602,22 @@ int sbk_create(const sbk_config *cfg, sbk_engine **out)
 int sbk_destroy(sbk_engine *e)
 {
     if (!e) return 0;
-    hipSetDevice(e->device);
-    if (e->stream) hipStreamSynchronize(e->stream);
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
     free_lists(e);
     for (auto &p : e->ev) {
-        hipEventDestroy(p.first);
-        hipEventDestroy(p.second);
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
     }
-    hipFree(e->d_f64);
-    hipFree(e->d_qf);
-    hipFree(e->d_cloud_of);
-    hipFree(e->d_cp);
-    hipFree(e->d_params);
-    hipFree(e->d_state);
-    if (e->h_state) hipHostFree(e->h_state);
-    if (e->stream) hipStreamDestroy(e->stream);
+    (void)hipFree(e->d_f64);
+    (void)hipFree(e->d_qf);
+    (void)hipFree(e->d_blocks);
+    (void)hipFree(e->d_errb);
+    (void)hipFree(e->d_cp);
+    (void)hipFree(e->d_params);
+    (void)hipFree(e->d_state);
+    if (e->h_state) (void)hipHostFree(e->h_state);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return 0;
 }
@@ -622,7 +656,7 @@ int sbk_set_fields(sbk_engine *e, int cloud, const sbk_fields *f)
     c.f = *f;
     if (!e->finalized) return 0;
     HIP_TRY(hipSetDevice(e->device));
-    sbk::CloudParams p = make_params(c.f, c.off, c.n);
+    sbk::CloudParams p = cloud_params(e, cloud);
     HIP_TRY(hipMemcpyAsync(e->d_params + cloud, &p, sizeof(p), hipMemcpyHostToDevice, e->stream));
     if (scale_changed && c.n > 0) {
         std::vector<double> qm(c.n), kc(c.n), qd(c.n);
@@ -884,8 +918,8 @@ int sbk_time_coulomb(sbk_engine *e, int precision, int reps, double *ms_per_laun
     HIP_TRY(hipEventSynchronize(t1));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
     *ms_per_launch = ms / reps;
     return 0;
 }
@@ -900,7 +934,7 @@ int sbk_autotune(sbk_engine *e, int precision, int *tpt_out, int *slices_out, co
         return 0;
     }
     const int tpts[3] = {1, 2, 4};
-    const int smul[6] = {0, 1, 2, 4, 8, 16};
+    const int rounds[7] = {0, 1, 2, 3, 4, 6, 8};   // 0: automatic split
     int best_t = e->tpt, best_s = e->slices_req;
     double best = 1e300;
     char dev_name[128] = {0};
@@ -908,19 +942,13 @@ int sbk_autotune(sbk_engine *e, int precision, int *tpt_out, int *slices_out, co
     FILE *fp = table ? std::fopen(table, "a") : nullptr;
     const int old_t = e->tpt, old_s = e->slices_req;
     for (int t : tpts) {
-        for (int m : smul) {
-            int slices = 0;
-            if (m > 0) {
-                // explicit slice count: m x the automatic one at this shape
-                e->tpt = t;
-                e->slices_req = 0;
-                ItemList *L = nullptr;
-                rc = get_list(e, precision, false, &L);
-                if (rc) break;
-                int base = 1;
-                for (int s : L->slices) base = std::max(base, s);
-                slices = std::max(1, base * m / 4);
-            }
+        long long blocks = 0;
+        for (auto &c : e->clouds)
+            if (c.n > 0) blocks += (c.n + sbk::kBlock * t - 1) / (sbk::kBlock * t);
+        const long long capacity =
+            (long long)sbk::coulomb_blocks_per_cu(precision, t, e->unit_charges ? 1 : 0) * e->cus;
+        for (int k : rounds) {
+            const int slices = k == 0 ? 0 : (int)std::max(1LL, (k * capacity + blocks / 2) / std::max(1LL, blocks));
             e->tpt = t;
             e->slices_req = slices;
             double ms = 0.0;

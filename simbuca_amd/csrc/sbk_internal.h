@@ -10,7 +10,10 @@
 //                    KV[1] aliases V0, and KV[7] is the new velocity vel2)
 //   KA[s][3][stride] stage accelerations, s=1..7 (k_s[3..5], bin/ode.cpp:104-180)
 //   CP[slice][3][stride] Coulomb partial sums per source slice
-//   ERR[stride]      per-ion squared scaled error of the embedded pair
+//   ERRB[nblocks]    per-ion-block sum of squared scaled errors of the embedded pair
+// Per-ion kernels run over a table of ion blocks (at most 256 ions of one
+// cloud each), so a block never straddles two clouds and stage 7 can reduce
+// its error terms in LDS before the per-cloud controller sums the blocks.
 // All FP64, structure of arrays, so every per-ion kernel reads and writes
 // unit-stride, fully coalesced lines.
 #ifndef SBK_INTERNAL_H
@@ -38,6 +41,11 @@ struct CloudParams {
     double delta;
     // layout
     int off, n, slices, pad;
+    int blk0, nblk;           // this cloud's range in the ion-block table
+};
+
+struct IonBlock {
+    int cloud, start, end, pad;
 };
 
 struct CloudState {
@@ -65,6 +73,7 @@ struct CoulombArgs {
     const CoulombItem *items;
     int nitems;
     int self_skip;                // FP64-exact: skip source index == target index
+    int unit_weights;             // every source weight is exactly 1 (charge +1, mj = 1)
     const CloudState *state;      // nullptr: every item active
     float eps_f;
     double eps_d;
@@ -75,11 +84,12 @@ struct EngineDev {
     double *KV[8][3];
     double *KA[8][3];
     double *CP;
-    double *ERR;
+    double *ERRB;
     double *derivs_out;           // [3][stride] scratch for sbk_eval_force/coulomb
     long long stride;
     const double *qm;             // charge*e/mass (bin/force.cpp:71)
-    const int *cloud_of;
+    const IonBlock *blocks;
+    int nblocks;
     const CloudParams *params;
     CloudState *state;
     const int *slices;            // per-cloud slice count of the current item list
@@ -93,6 +103,8 @@ enum KernelPrecision { KP_FP64_EXACT = 0, KP_FP64 = 1, KP_MIXED = 2, KP_MIXED_KA
 
 int launch_coulomb(int precision, int targets_per_thread, const CoulombArgs &a,
                    void *stream);
+// resident work-groups per CU of the Coulomb kernel a launch would use
+int coulomb_blocks_per_cu(int precision, int targets_per_thread, int unit_weights);
 int launch_stage(int stage, const EngineDev &d, void *stream);
 int launch_controller(const EngineDev &d, void *stream);
 int launch_commit(const EngineDev &d, void *stream);

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(kBlock) void coulomb_fp64_exact(CoulombArgs a)
 // FP64: the same tiling in FP64 with 1/sqrt.
 // The self pair has r = 0 and contributes exactly 0 through the eps term.
 // ---------------------------------------------------------------------------
-template <int R, int MODE>
+template <int R, int MODE, bool UNIT>
 __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
 {
     __shared__ double lx[kBlock], ly[kBlock], lz[kBlock];
@@ -167,7 +167,8 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
                         d2 = fmaf(dy, dy, d2);
                         d2 = fmaf(dx, dx, d2);
                         const float inv = __builtin_amdgcn_rsqf(d2);
-                        const float w = (q * inv) * (inv * inv);
+                        // UNIT: q == 1 exactly, and (1*inv)*(inv*inv) == inv*(inv*inv)
+                        const float w = UNIT ? inv * (inv * inv) : (q * inv) * (inv * inv);
                         ax[r] = fmaf(dx, w, ax[r]);
                         ay[r] = fmaf(dy, w, ay[r]);
                         az[r] = fmaf(dz, w, az[r]);
@@ -326,11 +327,12 @@ __device__ __forceinline__ void coulomb_total(const EngineDev &d, const CloudPar
 template <int S>
 __global__ __launch_bounds__(256) void stage_kernel(EngineDev d)
 {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= d.ntot) return;
-    const int c = d.cloud_of[g];
+    const IonBlock blk = d.blocks[blockIdx.x];
+    const int c = blk.cloud;
     const CloudState &cs = d.state[c];
-    if (!cs.active) return;
+    if (!cs.active) return;                       // uniform over the block
+    const bool valid = blk.start + (int)threadIdx.x < blk.end;
+    const long long g = valid ? blk.start + threadIdx.x : blk.start;
     const CloudParams &p = d.params[c];
     const double h = cs.h, t = cs.t;
     const double ts = d.time_mode ? t + kC[S] * h : t;
@@ -344,16 +346,20 @@ __global__ __launch_bounds__(256) void stage_kernel(EngineDev d)
     ion_accel(p, t, ts, d.qm[g], x, y, z, vx, vy, vz, Fx, Fy, Fz, a3[0], a3[1], a3[2]);
     const double v3[3] = {vx, vy, vz};
     const double s3[3] = {x, y, z};
+    if (valid) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) d.KA[S][k][g] = a3[k];
+        for (int k = 0; k < 3; ++k) d.KA[S][k][g] = a3[k];
+    }
 
     if constexpr (S < 7) {
+        if (!valid) return;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             d.PS[k][g] = d.P0[k][g] + h * combo<S, S + 1>(d.KV, k, g, v3[k]);
             d.KV[S + 1][k][g] = d.V0[k][g] + h * combo<S, S + 1>(d.KA, k, g, a3[k]);
         }
     } else {
+        __shared__ double red[256];
         // y_{n+1} equals the stage-7 state: a_7j = b_j, same evaluation order.
         double err = 0.0;
         bool first = true;
@@ -382,7 +388,13 @@ __global__ __launch_bounds__(256) void stage_kernel(EngineDev d)
                 first = false;
             }
         }
-        d.ERR[g] = err;
+        red[threadIdx.x] = valid ? err : 0.0;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) d.ERRB[blockIdx.x] = red[0];
     }
 }
 
@@ -401,7 +413,7 @@ __global__ __launch_bounds__(256) void controller_kernel(EngineDev d)
     }
     const CloudParams &p = d.params[c];
     double sum = 0.0;
-    for (int i = threadIdx.x; i < p.n; i += 256) sum += d.ERR[p.off + i];
+    for (int b = threadIdx.x; b < p.nblk; b += 256) sum += d.ERRB[p.blk0 + b];
     red[threadIdx.x] = sum;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -456,10 +468,10 @@ __global__ __launch_bounds__(256) void controller_kernel(EngineDev d)
 // (bin/ode.cpp:488-489).
 __global__ __launch_bounds__(256) void commit_kernel(EngineDev d)
 {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= d.ntot) return;
-    const int c = d.cloud_of[g];
-    if (!d.state[c].accepted_last) return;
+    const IonBlock blk = d.blocks[blockIdx.x];
+    if (!d.state[blk.cloud].accepted_last) return;
+    const long long g = blk.start + threadIdx.x;
+    if (g >= blk.end) return;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         d.P0[k][g] = d.PS[k][g];
@@ -482,9 +494,10 @@ __global__ void begin_kernel(EngineDev d, double duration, int budget, int set_e
 // Coulomb sum f alone.
 __global__ __launch_bounds__(256) void eval_derivs_kernel(EngineDev d, int with_fields)
 {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= d.ntot) return;
-    const int c = d.cloud_of[g];
+    const IonBlock blk = d.blocks[blockIdx.x];
+    const long long g = blk.start + threadIdx.x;
+    if (g >= blk.end) return;
+    const int c = blk.cloud;
     const CloudParams &p = d.params[c];
     double Fx, Fy, Fz;
     if (with_fields) {
@@ -512,24 +525,63 @@ static inline int check_launch()
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <int R>
-static int launch_tiled(int precision, const CoulombArgs &a, hipStream_t s)
+template <int R, bool UNIT>
+static int launch_tiled_u(int precision, const CoulombArgs &a, hipStream_t s)
 {
     switch (precision) {
     case KP_FP64:
-        hipLaunchKernelGGL((coulomb_tiled<R, KP_FP64>), dim3(a.nitems), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((coulomb_tiled<R, KP_FP64, false>), dim3(a.nitems), dim3(kBlock), 0, s,
+                           a);
         break;
     case KP_MIXED:
-        hipLaunchKernelGGL((coulomb_tiled<R, KP_MIXED>), dim3(a.nitems), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((coulomb_tiled<R, KP_MIXED, UNIT>), dim3(a.nitems), dim3(kBlock), 0, s,
+                           a);
         break;
     case KP_MIXED_KAHAN:
-        hipLaunchKernelGGL((coulomb_tiled<R, KP_MIXED_KAHAN>), dim3(a.nitems), dim3(kBlock), 0,
-                           s, a);
+        hipLaunchKernelGGL((coulomb_tiled<R, KP_MIXED_KAHAN, UNIT>), dim3(a.nitems), dim3(kBlock),
+                           0, s, a);
         break;
     default:
         return -1;
     }
     return check_launch();
+}
+
+template <int R>
+static int launch_tiled(int precision, const CoulombArgs &a, hipStream_t s)
+{
+    return a.unit_weights ? launch_tiled_u<R, true>(precision, a, s)
+                          : launch_tiled_u<R, false>(precision, a, s);
+}
+
+template <int R, bool UNIT>
+static const void *tiled_fn(int precision)
+{
+    switch (precision) {
+    case KP_FP64: return reinterpret_cast<const void *>(&coulomb_tiled<R, KP_FP64, false>);
+    case KP_MIXED: return reinterpret_cast<const void *>(&coulomb_tiled<R, KP_MIXED, UNIT>);
+    case KP_MIXED_KAHAN:
+        return reinterpret_cast<const void *>(&coulomb_tiled<R, KP_MIXED_KAHAN, UNIT>);
+    default: return nullptr;
+    }
+}
+
+int coulomb_blocks_per_cu(int precision, int tpt, int unit)
+{
+    const void *f = nullptr;
+    if (precision == KP_FP64_EXACT)
+        f = reinterpret_cast<const void *>(&coulomb_fp64_exact);
+    else if (tpt == 1)
+        f = unit ? tiled_fn<1, true>(precision) : tiled_fn<1, false>(precision);
+    else if (tpt == 2)
+        f = unit ? tiled_fn<2, true>(precision) : tiled_fn<2, false>(precision);
+    else if (tpt == 4)
+        f = unit ? tiled_fn<4, true>(precision) : tiled_fn<4, false>(precision);
+    if (!f) return 1;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kBlock, 0) != hipSuccess || nb < 1)
+        return 1;
+    return nb;
 }
 
 int launch_coulomb(int precision, int tpt, const CoulombArgs &a, void *stream)
@@ -551,7 +603,8 @@ int launch_coulomb(int precision, int tpt, const CoulombArgs &a, void *stream)
 int launch_stage(int stage, const EngineDev &d, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
-    const dim3 grid((unsigned)((d.ntot + 255) / 256)), block(256);
+    if (d.nblocks <= 0) return 0;
+    const dim3 grid((unsigned)d.nblocks), block(256);
     switch (stage) {
     case 1: hipLaunchKernelGGL(stage_kernel<1>, grid, block, 0, s, d); break;
     case 2: hipLaunchKernelGGL(stage_kernel<2>, grid, block, 0, s, d); break;
@@ -573,7 +626,8 @@ int launch_controller(const EngineDev &d, void *stream)
 
 int launch_commit(const EngineDev &d, void *stream)
 {
-    hipLaunchKernelGGL(commit_kernel, dim3((unsigned)((d.ntot + 255) / 256)), dim3(256), 0,
+    if (d.nblocks <= 0) return 0;
+    hipLaunchKernelGGL(commit_kernel, dim3((unsigned)d.nblocks), dim3(256), 0,
                        (hipStream_t)stream, d);
     return check_launch();
 }
@@ -587,7 +641,8 @@ int launch_begin(const EngineDev &d, double duration, int budget, int set_end, v
 
 int launch_eval_derivs(const EngineDev &d, int with_fields, void *stream)
 {
-    hipLaunchKernelGGL(eval_derivs_kernel, dim3((unsigned)((d.ntot + 255) / 256)), dim3(256), 0,
+    if (d.nblocks <= 0) return 0;
+    hipLaunchKernelGGL(eval_derivs_kernel, dim3((unsigned)d.nblocks), dim3(256), 0,
                        (hipStream_t)stream, d, with_fields);
     return check_launch();
 }

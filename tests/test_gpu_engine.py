@@ -299,3 +299,8 @@ def test_cunbody_abi_fp64_and_mixed(rng):
     amx = S.cunbody1_force(xj, mj, xi, 1e-20, precision="mixed")
     assert _rows_error(a64, ref) < 1e-13
     assert _rows_error(amx, ref) < 1e-6
+    # unit weights (what the reference passes, bin/cunbody.cpp:34-35) take the q == 1 kernel
+    ones = np.ones(nj)
+    ref1 = O.cunbody_force(xj, ones, xi, 1e-20)
+    assert _rows_error(S.cunbody1_force(xj, ones, xi, 1e-20, precision="mixed"), ref1) < 1e-6
+    assert _rows_error(S.cunbody1_force(xj, ones, xi, 1e-20, precision="mixed_kahan"), ref1) < 1e-6
