@@ -162,12 +162,14 @@ def _device_copy(nbytes, reps):
     hip.hipFree(dst)
 
 
-def kernels_only(S, w):
-    """For counter runs: 5 mixed and 2 FP64 Coulomb launches at the default
-    launch shape, then 3 device copies of 1 GiB whose traffic is known."""
+def kernels_only(S, w, shape=None):
+    """For counter runs: 5 mixed and 2 FP64 Coulomb launches (automatic or
+    given launch shape), then 3 device copies of 1 GiB whose traffic is known."""
     c = w.clouds[0]
     e = S.Engine(precision="mixed")
     e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
+    if shape is not None:
+        e.set_launch(*shape)
     log(f"mixed coulomb {e.time_coulomb('mixed', 5):.4f} ms/launch")
     log(f"fp64 coulomb {e.time_coulomb('fp64', 2):.4f} ms/launch")
     e.close()
@@ -179,7 +181,7 @@ def other_workload(S, w, args, precision):
     """Secondary workloads: attempt throughput of the mixed and FP64 engines
     (and optionally the whole simulated interval); one JSON line."""
     sizes = [cl.n for cl in w.clouds]
-    tune = None
+    tune = tuple(int(x) for x in args.shape.split(",")) if args.shape else None
     if args.tune:
         e = make_engine(S, w, precision, timing=False)
         tune = e.autotune(precision)
@@ -212,7 +214,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--fp64-steps", type=int, default=0, help="attempts timed for the FP64 engine")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tune", action="store_true", help="autotune the Coulomb launch shape first")
+    ap.add_argument("--no-tune", dest="tune", action="store_false",
+                    help="skip the autotuner and use the automatic launch shape")
+    ap.add_argument("--shape", default="",
+                    help="fixed Coulomb launch shape R,Q,S (targets/thread, source groups, "
+                         "slices); skips the autotuner")
     ap.add_argument("--n", type=int, default=0, help="override the ion count (testing)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the Coulomb kernels and a 1 GiB device copy (profiler runs)")
@@ -238,20 +244,22 @@ def main():
     info = S.device_info()
     log(f"device {info}; workload {w.name}; n={n}")
 
+    fixed = tuple(int(x) for x in args.shape.split(",")) if args.shape else None
+    if fixed is not None:
+        args.tune = False
     if args.kernels_only:
-        kernels_only(S, w)
+        kernels_only(S, w, fixed)
         return
     if args.workload != 3:
         other_workload(S, w, args, precision)
         return
 
-    tune = None
+    tune = fixed
     if args.tune:
-        e = S.Engine(precision="mixed")
-        e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
-        tune = e.autotune("mixed", os.path.join(ROOT, "gpurun_out", "tune_table.tsv"))
+        e = make_engine(S, w, precision, timing=False)
+        tune = e.autotune(precision)
         e.close()
-        log(f"autotuned launch shape (targets/thread, slices) = {tune}")
+        log(f"autotuned launch shape (targets/thread, source groups, slices) = {tune}")
 
     # mixed engine, timed region with HIP events around every Coulomb launch
     e, dt, st, status = run_engine(S, w, "mixed", args.steps, args.warmup, timing=True, tune=tune)

@@ -169,18 +169,21 @@ int  sbk_eval_force(sbk_engine *e, int cloud, double *derivs);
  * computed with `precision`: out[n][3]. */
 int  sbk_eval_coulomb(sbk_engine *e, int cloud, int precision, double *out);
 
-/* Coulomb launch shape.  targets_per_thread in {1,2,4}; slices = number of
- * source slices per target block (0 = automatic).  Applies to SBK_FP64 and
- * SBK_MIXED*; SBK_FP64_EXACT always uses one slice. */
-int  sbk_set_launch(sbk_engine *e, int targets_per_thread, int slices);
+/* Coulomb launch shape.  targets_per_thread in {1,2,4}; source_groups in
+ * {1,4}: wave groups of a 256-thread work-group that share one set of
+ * targets and split each source tile; slices = number of source slices per
+ * target block (0 = automatic).  Applies to SBK_FP64 and SBK_MIXED*;
+ * SBK_FP64_EXACT always uses one thread per target and one slice. */
+int  sbk_set_launch(sbk_engine *e, int targets_per_thread, int source_groups, int slices);
 /* Times `reps` launches of the Coulomb kernel of the current state's
  * positions for every cloud; ms_per_launch receives the mean. */
 int  sbk_time_coulomb(sbk_engine *e, int precision, int reps, double *ms_per_launch);
 /* Tries the launch shapes for this engine's clouds and keeps the fastest;
- * returns the chosen shape.  The table of results is appended to
+ * returns the chosen shape.  One line per shape tried ("device ions clouds
+ * precision targets_per_thread source_groups slices ms") is appended to
  * `table_path` if non-null. */
-int  sbk_autotune(sbk_engine *e, int precision, int *targets_per_thread, int *slices,
-                  const char *table_path);
+int  sbk_autotune(sbk_engine *e, int precision, int *targets_per_thread, int *source_groups,
+                  int *slices, const char *table_path);
 
 const char *sbk_last_error(void);
 /* Device name, compute-unit count and clock (MHz) of the current device. */

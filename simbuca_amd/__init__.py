@@ -147,9 +147,9 @@ def lib():
         "sbk_reset_stats": (c_int, [vp]),
         "sbk_eval_force": (c_int, [vp, c_int, dp]),
         "sbk_eval_coulomb": (c_int, [vp, c_int, c_int, dp]),
-        "sbk_set_launch": (c_int, [vp, c_int, c_int]),
+        "sbk_set_launch": (c_int, [vp, c_int, c_int, c_int]),
         "sbk_time_coulomb": (c_int, [vp, c_int, c_int, dp]),
-        "sbk_autotune": (c_int, [vp, c_int, ip, ip, ctypes.c_char_p]),
+        "sbk_autotune": (c_int, [vp, c_int, ip, ip, ip, ctypes.c_char_p]),
         "sbk_last_error": (ctypes.c_char_p, []),
         "sbk_device_info": (c_int, [ctypes.c_char_p, c_int, ip, ip]),
     }
@@ -249,8 +249,9 @@ class Engine:
     def set_step(self, cloud, h):
         _check(lib().sbk_set_step(self._h, cloud, float(h)))
 
-    def set_launch(self, targets_per_thread=2, slices=0):
-        _check(lib().sbk_set_launch(self._h, int(targets_per_thread), int(slices)))
+    def set_launch(self, targets_per_thread=2, source_groups=4, slices=0):
+        _check(lib().sbk_set_launch(self._h, int(targets_per_thread), int(source_groups),
+                                    int(slices)))
 
     def advance(self, duration, max_attempts=1 << 62):
         _check(lib().sbk_advance(self._h, float(duration), int(max_attempts)))
@@ -306,10 +307,10 @@ class Engine:
 
     def autotune(self, precision="mixed", table_path=None):
         p = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
-        t = ctypes.c_int(); s = ctypes.c_int()
-        _check(lib().sbk_autotune(self._h, p, ctypes.byref(t), ctypes.byref(s),
+        t = ctypes.c_int(); q = ctypes.c_int(); s = ctypes.c_int()
+        _check(lib().sbk_autotune(self._h, p, ctypes.byref(t), ctypes.byref(q), ctypes.byref(s),
                                   table_path.encode() if table_path else None))
-        return t.value, s.value
+        return t.value, q.value, s.value
 
 
 def cunbody1_force(xj, mj, xi, eps2, precision=None):

@@ -90,9 +90,9 @@ void ensure(int ni, int nj)
     }
 }
 
-void build_items(int ni, int nj, int tpt)
+void build_items(int ni, int nj, int tpt, int q)
 {
-    const int blk = sbk::kBlock * tpt;
+    const int blk = sbk::kBlock / q * tpt;
     const int ib = (ni + blk - 1) / blk;
     int S = 1;
     if (g.precision != SBK_FP64_EXACT) {
@@ -182,8 +182,9 @@ void cunbody1_force_begin(double xj[][3], double mj[], double xi[][3], double ep
     for (int j = 0; j < nj; ++j) mf[j] = (float)mj[j];
     CHK(hipMemcpyAsync(g.d_mf, mf.data(), sizeof(float) * nj, hipMemcpyHostToDevice, g.stream));
     const int tpt = (g.precision == SBK_FP64_EXACT) ? 1 : 2;
-    build_items(ni, nj, tpt);   // synchronizes: mf is consumed before it goes out of scope
-    const int blk = sbk::kBlock * tpt;
+    const int q = (g.precision == SBK_FP64_EXACT) ? 1 : 4;
+    build_items(ni, nj, tpt, q);   // synchronizes: mf is consumed before it goes out of scope
+    const int blk = sbk::kBlock / q * tpt;
     const int S = g.nitems / ((ni + blk - 1) / blk);   // items = slices x target blocks
     const long long stride = ci;
     if ((long long)S * 3 * stride > partials_cap) {
@@ -211,7 +212,7 @@ void cunbody1_force_begin(double xj[][3], double mj[], double xi[][3], double ep
     a.state = nullptr;
     a.eps_d = eps2;
     a.eps_f = std::max((float)eps2, 1e-23f);   // FP32 needs eps >= 1e-23 to keep r=0 finite
-    if (sbk::launch_coulomb(g.precision, tpt, a, g.stream)) die("Coulomb kernel launch", hipGetLastError());
+    if (sbk::launch_coulomb(g.precision, tpt, q, a, g.stream)) die("Coulomb kernel launch", hipGetLastError());
     hipLaunchKernelGGL(reduce_slices, dim3((ni + 255) / 256), dim3(256), 0, g.stream,
                        (const double *)partials, stride, S, ni, dai);
     CHK(hipGetLastError());

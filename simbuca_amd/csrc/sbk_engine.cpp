@@ -96,9 +96,10 @@ struct sbk_engine {
     sbk::EngineDev dev{};
 
     // launch shape
-    int tpt = 2;
+    int tpt = 4;
+    int groups = 4;
     int slices_req = 0;
-    std::map<std::tuple<int, int, int, int>, ItemList> lists;
+    std::map<std::tuple<int, int, int, int, int>, ItemList> lists;
 
     // graph of one attempt
     hipGraph_t graph = nullptr;
@@ -174,23 +175,25 @@ void free_lists(sbk_engine *e)
 // launch shape.  `only_coulomb` keeps the clouds whose Coulomb switch is on.
 int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
 {
-    const int tpt = (precision == SBK_FP64_EXACT) ? 1 : e->tpt;
-    const int sreq = (precision == SBK_FP64_EXACT) ? 1 : e->slices_req;
-    auto key = std::make_tuple(precision, tpt, sreq, only_coulomb ? 1 : 0);
+    const bool exact = precision == SBK_FP64_EXACT;
+    const int tpt = exact ? 1 : e->tpt;
+    const int q = exact ? 1 : e->groups;
+    const int sreq = exact ? 1 : e->slices_req;
+    auto key = std::make_tuple(precision, tpt, q, sreq, only_coulomb ? 1 : 0);
     auto it = e->lists.find(key);
     if (it != e->lists.end()) {
         *out = &it->second;
         return 0;
     }
-    const int blk = sbk::kBlock * tpt;
+    const int blk = sbk::kBlock / q * tpt;   // targets per work-group
     long long total_blocks = 0;
     for (auto &c : e->clouds)
         if (c.n > 0 && (!only_coulomb || c.f.coulomb)) total_blocks += (c.n + blk - 1) / blk;
-    // automatic split: two full rounds of resident work-groups, so that no
-    // partially filled last round (tail) is left on the CUs
+    // automatic split: about eight rounds of resident work-groups; short
+    // items keep the last, partially filled round (the tail) short
     const long long capacity =
-        (long long)sbk::coulomb_blocks_per_cu(precision, tpt, e->unit_charges ? 1 : 0) * e->cus;
-    const long long target = 2 * capacity;
+        (long long)sbk::coulomb_blocks_per_cu(precision, tpt, e->unit_charges ? 1 : 0, q) * e->cus;
+    const long long target = 8 * capacity;
     ItemList L;
     std::vector<sbk::CoulombItem> items;
     L.slices.assign(e->clouds.size(), 1);
@@ -204,7 +207,9 @@ int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
                 S = sreq;
             else {
                 long long want = total_blocks > 0 ? (target + total_blocks / 2) / total_blocks : 1;
-                long long cap = std::max(1, c.n / 512);
+                // at least 512 sources per slice (partial sums stay a few % of the traffic),
+                // relaxed to 64 for clouds too small to fill the device otherwise
+                long long cap = std::max(1, c.n / (total_blocks * 8 < capacity ? 64 : 512));
                 S = (int)std::max(1LL, std::min(want, cap));
             }
         }
@@ -437,7 +442,7 @@ int launch_coulomb_timed(sbk_engine *e, int precision, const sbk::CoulombArgs &a
     }
     sbk::CoulombArgs b = a;
     if (precision == SBK_FP64) b.eps_d = 1e-40;   // the guard of bin/force.cpp:136
-    if (sbk::launch_coulomb(kp(precision), e->tpt, b, e->stream))
+    if (sbk::launch_coulomb(kp(precision), e->tpt, e->groups, b, e->stream))
         return fail(SBK_EHIP, "Coulomb kernel launch failed: %s",
                     hipGetErrorString(hipGetLastError()));
     if (evp) HIP_TRY(hipEventRecord(evp->second, e->stream));
@@ -882,13 +887,15 @@ int sbk_eval_coulomb(sbk_engine *e, int cloud, int precision, double *out)
     return eval_common(e, cloud, precision, 0, out);
 }
 
-int sbk_set_launch(sbk_engine *e, int tpt, int slices)
+int sbk_set_launch(sbk_engine *e, int tpt, int groups, int slices)
 {
     if (!e) return fail(SBK_EINVAL, "null engine");
     if (tpt != 1 && tpt != 2 && tpt != 4) return fail(SBK_EINVAL, "targets_per_thread must be 1, 2 or 4");
+    if (groups != 1 && groups != 4) return fail(SBK_EINVAL, "source_groups must be 1 or 4");
     if (slices < 0) return fail(SBK_EINVAL, "slices must be >= 0");
-    if (tpt != e->tpt || slices != e->slices_req) {
+    if (tpt != e->tpt || groups != e->groups || slices != e->slices_req) {
         e->tpt = tpt;
+        e->groups = groups;
         e->slices_req = slices;
         destroy_graph(e);
     }
@@ -912,7 +919,7 @@ int sbk_time_coulomb(sbk_engine *e, int precision, int reps, double *ms_per_laun
     HIP_TRY(hipEventCreate(&t1));
     HIP_TRY(hipEventRecord(t0, e->stream));
     for (int i = 0; i < reps; ++i)
-        if (sbk::launch_coulomb(kp(precision), e->tpt, a, e->stream))
+        if (sbk::launch_coulomb(kp(precision), e->tpt, e->groups, a, e->stream))
             return fail(SBK_EHIP, "Coulomb kernel launch failed");
     HIP_TRY(hipEventRecord(t1, e->stream));
     HIP_TRY(hipEventSynchronize(t1));
@@ -924,56 +931,69 @@ int sbk_time_coulomb(sbk_engine *e, int precision, int reps, double *ms_per_laun
     return 0;
 }
 
-int sbk_autotune(sbk_engine *e, int precision, int *tpt_out, int *slices_out, const char *table)
+int sbk_autotune(sbk_engine *e, int precision, int *tpt_out, int *groups_out, int *slices_out,
+                 const char *table)
 {
     int rc = prepare(e);
     if (rc) return rc;
     if (precision == SBK_FP64_EXACT) {
         if (tpt_out) *tpt_out = 1;
+        if (groups_out) *groups_out = 1;
         if (slices_out) *slices_out = 1;
         return 0;
     }
     const int tpts[3] = {1, 2, 4};
-    const int rounds[7] = {0, 1, 2, 3, 4, 6, 8};   // 0: automatic split
-    int best_t = e->tpt, best_s = e->slices_req;
+    const int qs[2] = {1, 4};
+    const int rounds[6] = {0, 1, 2, 3, 4, 8};   // 0: automatic split
+    int best_t = e->tpt, best_q = e->groups, best_s = e->slices_req;
     double best = 1e300;
     char dev_name[128] = {0};
     sbk_device_info(dev_name, sizeof(dev_name), nullptr, nullptr);
     FILE *fp = table ? std::fopen(table, "a") : nullptr;
-    const int old_t = e->tpt, old_s = e->slices_req;
+    const int old_t = e->tpt, old_q = e->groups, old_s = e->slices_req;
     for (int t : tpts) {
-        long long blocks = 0;
-        for (auto &c : e->clouds)
-            if (c.n > 0) blocks += (c.n + sbk::kBlock * t - 1) / (sbk::kBlock * t);
-        const long long capacity =
-            (long long)sbk::coulomb_blocks_per_cu(precision, t, e->unit_charges ? 1 : 0) * e->cus;
-        for (int k : rounds) {
-            const int slices = k == 0 ? 0 : (int)std::max(1LL, (k * capacity + blocks / 2) / std::max(1LL, blocks));
-            e->tpt = t;
-            e->slices_req = slices;
-            double ms = 0.0;
-            rc = sbk_time_coulomb(e, precision, 1, &ms);   // warm-up and list build
-            if (rc) break;
-            rc = sbk_time_coulomb(e, precision, 3, &ms);
-            if (rc) break;
-            if (fp)
-                std::fprintf(fp, "%s\t%d\t%zu\t%d\t%d\t%d\t%.6f\n", dev_name, e->ntot,
-                             e->clouds.size(), precision, t, slices, ms);
-            if (ms < best) {
-                best = ms;
-                best_t = t;
-                best_s = slices;
+        for (int q : qs) {
+            long long blocks = 0;
+            const int blk = sbk::kBlock / q * t;
+            for (auto &c : e->clouds)
+                if (c.n > 0) blocks += (c.n + blk - 1) / blk;
+            const long long capacity =
+                (long long)sbk::coulomb_blocks_per_cu(precision, t, e->unit_charges ? 1 : 0, q) *
+                e->cus;
+            for (int k : rounds) {
+                const int slices =
+                    k == 0 ? 0 : (int)std::max(1LL, (k * capacity + blocks / 2) / std::max(1LL, blocks));
+                e->tpt = t;
+                e->groups = q;
+                e->slices_req = slices;
+                double ms = 0.0;
+                rc = sbk_time_coulomb(e, precision, 1, &ms);   // warm-up and list build
+                if (rc) break;
+                rc = sbk_time_coulomb(e, precision, 3, &ms);
+                if (rc) break;
+                if (fp)
+                    std::fprintf(fp, "%s\t%d\t%zu\t%d\t%d\t%d\t%d\t%.6f\n", dev_name, e->ntot,
+                                 e->clouds.size(), precision, t, q, slices, ms);
+                if (ms < best) {
+                    best = ms;
+                    best_t = t;
+                    best_q = q;
+                    best_s = slices;
+                }
             }
+            if (rc) break;
         }
         if (rc) break;
     }
     if (fp) std::fclose(fp);
     e->tpt = old_t;
+    e->groups = old_q;
     e->slices_req = old_s;
     if (rc) return rc;
-    rc = sbk_set_launch(e, best_t, best_s);
+    rc = sbk_set_launch(e, best_t, best_q, best_s);
     if (rc) return rc;
     if (tpt_out) *tpt_out = best_t;
+    if (groups_out) *groups_out = best_q;
     if (slices_out) *slices_out = best_s;
     return 0;
 }

@@ -101,10 +101,11 @@ struct EngineDev {
 // host-side launchers (sbk_kernels.hip)
 enum KernelPrecision { KP_FP64_EXACT = 0, KP_FP64 = 1, KP_MIXED = 2, KP_MIXED_KAHAN = 3 };
 
-int launch_coulomb(int precision, int targets_per_thread, const CoulombArgs &a,
+// q: wave groups of a work-group that split the sources of one target set (1 or 4)
+int launch_coulomb(int precision, int targets_per_thread, int q, const CoulombArgs &a,
                    void *stream);
 // resident work-groups per CU of the Coulomb kernel a launch would use
-int coulomb_blocks_per_cu(int precision, int targets_per_thread, int unit_weights);
+int coulomb_blocks_per_cu(int precision, int targets_per_thread, int unit_weights, int q);
 int launch_stage(int stage, const EngineDev &d, void *stream);
 int launch_controller(const EngineDev &d, void *stream);
 int launch_commit(const EngineDev &d, void *stream);

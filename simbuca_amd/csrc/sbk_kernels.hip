@@ -81,11 +81,14 @@ __global__ __launch_bounds__(kBlock) void coulomb_fp64_exact(CoulombArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Coulomb direct sum, tiled.  A work-group owns kBlock*R targets (R per
-// thread, strided by kBlock so a wave's loads are unit stride) and one slice
-// of the sources.  Sources pass through LDS kBlock at a time; inside a tile
-// every lane reads the same source (an LDS broadcast) and updates its R
-// targets.
+// Coulomb direct sum, tiled.  A work-group owns (kBlock/Q)*R targets (R per
+// thread, strided so a wave's loads are unit stride) and one slice of the
+// sources.  Sources pass through LDS kBlock at a time; inside a tile every
+// lane of a wave reads the same source (an LDS broadcast) and updates its R
+// targets.  With Q > 1 the work-group's Q wave groups share one set of
+// targets and each takes 1/Q of every tile; their sums are combined in LDS in
+// a fixed order at the end, so a cloud yields Q times more work-groups
+// without more partial sums in memory.
 //
 // MIXED: separations are formed in FP64 and rounded to FP32; |r|^2, rsqrt
 // and the sum over kSubTile sources are FP32; the sub-tile sums are folded
@@ -93,9 +96,11 @@ __global__ __launch_bounds__(kBlock) void coulomb_fp64_exact(CoulombArgs a)
 // FP64: the same tiling in FP64 with 1/sqrt.
 // The self pair has r = 0 and contributes exactly 0 through the eps term.
 // ---------------------------------------------------------------------------
-template <int R, int MODE, bool UNIT>
+template <int R, int MODE, bool UNIT, int Q>
 __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
 {
+    constexpr int L = kBlock / Q;          // lanes that share one target set
+    constexpr int SPAN = kBlock / Q;       // sources of a tile handled by one wave group
     __shared__ double lx[kBlock], ly[kBlock], lz[kBlock];
     __shared__ float lqf[kBlock];
     __shared__ double lqd[(MODE == KP_FP64) ? kBlock : 1];
@@ -103,13 +108,14 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
     const CoulombItem it = a.items[xcd_item(blockIdx.x, gridDim.x)];
     if (a.state && !a.state[it.cloud].active) return;
     const int tid = threadIdx.x;
+    const int lane = tid % L, grp = tid / L;
 
     double px[R], py[R], pz[R];
     double Ax[R], Ay[R], Az[R];
     float Sx[R], Sy[R], Sz[R], Cx[R], Cy[R], Cz[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int g = it.i0 + r * kBlock + tid;
+        const int g = it.i0 + r * L + lane;
         const bool v = g < it.i1;
         px[r] = v ? a.tx[g] : 0.0;
         py[r] = v ? a.ty[g] : 0.0;
@@ -134,9 +140,11 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
                 lqf[tid] = a.sqf[t0 + tid];
         }
         __syncthreads();
+        // this wave group's share of the tile
+        const int b0 = grp * SPAN, b1 = min(cnt, b0 + SPAN);
 
         if (MODE == KP_FP64) {
-            for (int j = 0; j < cnt; ++j) {
+            for (int j = b0; j < b1; ++j) {
                 const double sx = lx[j], sy = ly[j], sz = lz[j], q = lqd[j];
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
@@ -150,11 +158,11 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
                 }
             }
         } else {
-            for (int jj = 0; jj < cnt; jj += kSubTile) {
+            for (int jj = b0; jj < b1; jj += kSubTile) {
                 float ax[R], ay[R], az[R];
 #pragma unroll
                 for (int r = 0; r < R; ++r) ax[r] = ay[r] = az[r] = 0.f;
-                const int m = min(kSubTile, cnt - jj);
+                const int m = min(kSubTile, b1 - jj);
                 auto body = [&](int j) {
                     const double sx = lx[j], sy = ly[j], sz = lz[j];
                     const float q = lqf[j];
@@ -197,15 +205,39 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
         }
     }
 
+    if (MODE == KP_MIXED_KAHAN) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            Ax[r] = (double)Sx[r] - (double)Cx[r];
+            Ay[r] = (double)Sy[r] - (double)Cy[r];
+            Az[r] = (double)Sz[r] - (double)Cz[r];
+        }
+    }
+    if (Q > 1) {
+        // combine the wave groups in a fixed order (group 0 first), one target
+        // set at a time through the (now free) source tile arrays
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            __syncthreads();
+            lx[tid] = Ax[r];
+            ly[tid] = Ay[r];
+            lz[tid] = Az[r];
+            __syncthreads();
+            if (grp == 0) {
+                for (int q = 1; q < Q; ++q) {
+                    Ax[r] += lx[q * L + lane];
+                    Ay[r] += ly[q * L + lane];
+                    Az[r] += lz[q * L + lane];
+                }
+            }
+        }
+        if (grp != 0) return;
+    }
+
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int g = it.i0 + r * kBlock + tid;
+        const int g = it.i0 + r * L + lane;
         if (g < it.i1) {
-            if (MODE == KP_MIXED_KAHAN) {
-                Ax[r] = (double)Sx[r] - (double)Cx[r];
-                Ay[r] = (double)Sy[r] - (double)Cy[r];
-                Az[r] = (double)Sz[r] - (double)Cz[r];
-            }
             const double k = a.kc ? a.kc[g] : 1.0;
             double *o = a.out + (long long)it.slice * 3 * a.stride + g;
             o[0] = k * Ax[r];
@@ -525,58 +557,40 @@ static inline int check_launch()
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <int R, bool UNIT>
-static int launch_tiled_u(int precision, const CoulombArgs &a, hipStream_t s)
+template <int R, bool UNIT, int Q>
+static const void *tiled_fn_q(int precision)
 {
     switch (precision) {
-    case KP_FP64:
-        hipLaunchKernelGGL((coulomb_tiled<R, KP_FP64, false>), dim3(a.nitems), dim3(kBlock), 0, s,
-                           a);
-        break;
-    case KP_MIXED:
-        hipLaunchKernelGGL((coulomb_tiled<R, KP_MIXED, UNIT>), dim3(a.nitems), dim3(kBlock), 0, s,
-                           a);
-        break;
+    case KP_FP64: return reinterpret_cast<const void *>(&coulomb_tiled<R, KP_FP64, false, Q>);
+    case KP_MIXED: return reinterpret_cast<const void *>(&coulomb_tiled<R, KP_MIXED, UNIT, Q>);
     case KP_MIXED_KAHAN:
-        hipLaunchKernelGGL((coulomb_tiled<R, KP_MIXED_KAHAN, UNIT>), dim3(a.nitems), dim3(kBlock),
-                           0, s, a);
-        break;
-    default:
-        return -1;
-    }
-    return check_launch();
-}
-
-template <int R>
-static int launch_tiled(int precision, const CoulombArgs &a, hipStream_t s)
-{
-    return a.unit_weights ? launch_tiled_u<R, true>(precision, a, s)
-                          : launch_tiled_u<R, false>(precision, a, s);
-}
-
-template <int R, bool UNIT>
-static const void *tiled_fn(int precision)
-{
-    switch (precision) {
-    case KP_FP64: return reinterpret_cast<const void *>(&coulomb_tiled<R, KP_FP64, false>);
-    case KP_MIXED: return reinterpret_cast<const void *>(&coulomb_tiled<R, KP_MIXED, UNIT>);
-    case KP_MIXED_KAHAN:
-        return reinterpret_cast<const void *>(&coulomb_tiled<R, KP_MIXED_KAHAN, UNIT>);
+        return reinterpret_cast<const void *>(&coulomb_tiled<R, KP_MIXED_KAHAN, UNIT, Q>);
     default: return nullptr;
     }
 }
 
-int coulomb_blocks_per_cu(int precision, int tpt, int unit)
+template <int R>
+static const void *tiled_fn(int precision, int unit, int q)
 {
-    const void *f = nullptr;
-    if (precision == KP_FP64_EXACT)
-        f = reinterpret_cast<const void *>(&coulomb_fp64_exact);
-    else if (tpt == 1)
-        f = unit ? tiled_fn<1, true>(precision) : tiled_fn<1, false>(precision);
-    else if (tpt == 2)
-        f = unit ? tiled_fn<2, true>(precision) : tiled_fn<2, false>(precision);
-    else if (tpt == 4)
-        f = unit ? tiled_fn<4, true>(precision) : tiled_fn<4, false>(precision);
+    if (q == 4) return unit ? tiled_fn_q<R, true, 4>(precision) : tiled_fn_q<R, false, 4>(precision);
+    if (q == 1) return unit ? tiled_fn_q<R, true, 1>(precision) : tiled_fn_q<R, false, 1>(precision);
+    return nullptr;
+}
+
+static const void *coulomb_fn(int precision, int tpt, int unit, int q)
+{
+    if (precision == KP_FP64_EXACT) return reinterpret_cast<const void *>(&coulomb_fp64_exact);
+    switch (tpt) {
+    case 1: return tiled_fn<1>(precision, unit, q);
+    case 2: return tiled_fn<2>(precision, unit, q);
+    case 4: return tiled_fn<4>(precision, unit, q);
+    default: return nullptr;
+    }
+}
+
+int coulomb_blocks_per_cu(int precision, int tpt, int unit, int q)
+{
+    const void *f = coulomb_fn(precision, tpt, unit, q);
     if (!f) return 1;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kBlock, 0) != hipSuccess || nb < 1)
@@ -584,20 +598,16 @@ int coulomb_blocks_per_cu(int precision, int tpt, int unit)
     return nb;
 }
 
-int launch_coulomb(int precision, int tpt, const CoulombArgs &a, void *stream)
+int launch_coulomb(int precision, int tpt, int q, const CoulombArgs &a, void *stream)
 {
-    hipStream_t s = (hipStream_t)stream;
     if (a.nitems <= 0) return 0;
-    if (precision == KP_FP64_EXACT) {
-        hipLaunchKernelGGL(coulomb_fp64_exact, dim3(a.nitems), dim3(kBlock), 0, s, a);
-        return check_launch();
-    }
-    switch (tpt) {
-    case 1: return launch_tiled<1>(precision, a, s);
-    case 2: return launch_tiled<2>(precision, a, s);
-    case 4: return launch_tiled<4>(precision, a, s);
-    default: return -1;
-    }
+    const void *f = coulomb_fn(precision, tpt, a.unit_weights, q);
+    if (!f) return -1;
+    void *args[] = {const_cast<CoulombArgs *>(&a)};
+    if (hipLaunchKernel(f, dim3(a.nitems), dim3(kBlock), args, 0, (hipStream_t)stream) !=
+        hipSuccess)
+        return -1;
+    return check_launch();
 }
 
 int launch_stage(int stage, const EngineDev &d, void *stream)

@@ -83,3 +83,12 @@ def test_synth_workloads_shapes():
     w2 = synth.workload(2, n_override=4000)
     frac = np.mean(w2.clouds[0].mass_amu == 85.0)
     assert 0.45 < frac < 0.55
+
+
+def test_cxx_caller_links_against_the_header_only():
+    # built with g++ from include/simbuca_hip.h; loads the library, prints usage, no GPU work
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "cxx", "abi_caller")
+    assert os.path.exists(exe), "build tests/cxx (make -C tests/cxx)"
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 2 and "usage" in p.stderr

@@ -111,13 +111,14 @@ def test_coulomb_precisions_at_65536(precision, tol):
 
 
 @pytest.mark.parametrize("tpt", [1, 2, 4])
+@pytest.mark.parametrize("groups", [1, 4])
 @pytest.mark.parametrize("slices", [0, 1, 3, 7])
-def test_launch_shapes_agree(rng, tpt, slices):
+def test_launch_shapes_agree(rng, tpt, groups, slices):
     amu, m, ch, pos, vel = _rand_cloud(rng, 3001, sigma=1e-3)
     sf, _ = _both_fields(trap=0, coulomb=1)
     with S.Engine(precision="mixed") as e:
         e.add_cloud(pos, vel, m, ch, sf)
-        e.set_launch(tpt, slices)
+        e.set_launch(tpt, groups, slices)
         mixed = e.eval_coulomb(0, "mixed")
         fp64 = e.eval_coulomb(0, "fp64")
         exact = e.eval_coulomb(0, "fp64_exact")
@@ -237,6 +238,32 @@ def test_penning_physics_vs_oracle(rng):
     assert st["lifetime"] == pytest.approx(sc["lifetime"], rel=1e-10)
     assert np.allclose(gp, op, rtol=1e-9, atol=1e-9 * np.abs(op).max())
     assert np.allclose(gv, ov, rtol=1e-9, atol=1e-9 * np.abs(ov).max())
+
+
+def test_ensemble_clouds_have_independent_controllers(rng):
+    # uneven clouds, own start time, drive frequency and step-size history,
+    # advanced together; each must follow its own CPU trajectory
+    sizes, t0s, fds = [50, 200, 131], [0.0, 3e-7, 1.1e-6], [900.0, 2.0e5, 7.7e5]
+    clouds = []
+    with S.Engine(precision="fp64_exact", rtol=1e-9, atol=1e-9, time_mode=S.TIME_STAGE) as e:
+        for n, t0, fd in zip(sizes, t0s, fds):
+            amu, m, ch, pos, vel = _rand_cloud(rng, n, species=(133.0, 85.0), charges=(1,))
+            kw = dict(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0, Ed=30.0, fd=fd)
+            sf, of = _both_fields(**kw)
+            e.add_cloud(pos, vel, m, ch, sf, lifetime=t0)
+            clouds.append((pos, vel, m, ch, of, t0))
+        e.advance(1.5e-7)
+        got = [(*e.state(i), e.status(i)) for i in range(3)]
+    for (pos, vel, m, ch, of, t0), (gp, gv, st) in zip(clouds, got):
+        c = O.Cloud(pos, vel, m, ch, of, lifetime=t0, h0=1e-9, rtol=1e-9, atol=1e-9,
+                    time_mode=O.TIME_STAGE)
+        c.advance(t0 + 1.5e-7)
+        op, ov, sc = c.state()
+        assert st["attempts"] == sc["attempts"] and st["steps"] == sc["steps"]
+        assert st["lifetime"] == pytest.approx(sc["lifetime"], rel=1e-12)
+        assert st["lifetime"] >= t0 + 1.5e-7
+        assert np.allclose(gp, op, rtol=1e-10, atol=1e-10 * np.abs(op).max())
+        assert np.allclose(gv, ov, rtol=1e-10, atol=1e-10 * np.abs(ov).max())
 
 
 def test_mixed_short_horizon_deviation(rng):
