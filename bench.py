@@ -36,6 +36,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOP_PER_PAIR = 20.0          # GPU Gems 3 n-body convention
+# HBM bytes per launch of the mixed Coulomb kernel, from rocprofv3 FETCH_SIZE (x2.0) and
+# WRITE_SIZE (x1.0) counter runs (profiles/coulomb_mixed_65536_counters.md), keyed by
+# (architecture, ions, launch shape)
+MEASURED_TRAFFIC = {("gfx950", 65536, (4, 4, 0)): 17.43e6 + 50.33e6}
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector peak
 PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector peak
 HBM_PEAK_GBS = 8000.0
@@ -43,6 +47,41 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def tuned_shape(S, w, precision):
+    """Launch shape from the shipped per-GPU table (simbuca_amd/tuning/<arch>.tsv,
+    written by sbk_autotune); the autotuner runs only when no line matches."""
+    device = S.device_info()["name"]
+    table = os.path.join(ROOT, "simbuca_amd", "tuning", f"{device.split(':')[0]}.tsv")
+    shape = best_table_shape(table, device, w, precision) if os.path.exists(table) else None
+    if shape is not None:
+        log(f"launch shape {shape} from {os.path.relpath(table, ROOT)}")
+        return shape
+    e = make_engine(S, w, precision, timing=False)
+    try:
+        shape = e.autotune(precision)
+    finally:
+        e.close()
+    log(f"autotuned launch shape (targets/thread, source groups, slices) = {shape}")
+    return shape
+
+
+def best_table_shape(table, device, w, precision):
+    from simbuca_amd import PRECISIONS
+    n = sum(c.n for c in w.clouds)
+    best = None
+    with open(table) as fh:
+        for line in fh:
+            f = line.split()
+            if len(f) != 8 or f[0] != device:
+                continue
+            if int(f[1]) != n or int(f[2]) != len(w.clouds) or int(f[3]) != PRECISIONS[precision]:
+                continue
+            ms = float(f[7])
+            if best is None or ms < best[0]:
+                best = (ms, (int(f[4]), int(f[5]), int(f[6])))
+    return best[1] if best else None
 
 
 def make_engine(S, w, precision, timing, use_graph=True, tune=None):
@@ -168,10 +207,9 @@ def kernels_only(S, w, shape=None):
     c = w.clouds[0]
     e = S.Engine(precision="mixed")
     e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
-    if shape is not None:
-        e.set_launch(*shape)
-    log(f"mixed coulomb {e.time_coulomb('mixed', 5):.4f} ms/launch")
-    log(f"fp64 coulomb {e.time_coulomb('fp64', 2):.4f} ms/launch")
+    for p, reps in (("mixed", 5), ("fp64", 2)):
+        e.set_launch(*(shape or tuned_shape(S, w, p) or (4, 4, 0)))
+        log(f"{p} coulomb {e.time_coulomb(p, reps):.4f} ms/launch")
     e.close()
     _device_copy(1 << 30, 3)
     log("copied 3 x 1 GiB")
@@ -182,13 +220,12 @@ def other_workload(S, w, args, precision):
     (and optionally the whole simulated interval); one JSON line."""
     sizes = [cl.n for cl in w.clouds]
     tune = tuple(int(x) for x in args.shape.split(",")) if args.shape else None
-    if args.tune:
-        e = make_engine(S, w, precision, timing=False)
-        tune = e.autotune(precision)
-        e.close()
     res = {"workload": w.name, "clouds": len(sizes), "ions": int(sum(sizes)),
-           "pairs_per_attempt": 7.0 * sum(float(n) * (n - 1) for n in sizes), "tune": tune}
+           "pairs_per_attempt": 7.0 * sum(float(n) * (n - 1) for n in sizes), "tune": {}}
+    fixed = tune
     for p in (precision, "fp64"):
+        tune = fixed if fixed is not None else (tuned_shape(S, w, p) if args.tune else None)
+        res.setdefault("tune", {})[p] = tune
         e, dt, st, _ = run_engine(S, w, p, args.steps, args.warmup, timing=True, tune=tune)
         res[p] = {"attempts": st["attempts"], "steps": st["steps"], "seconds": dt,
                   "ms_per_attempt": 1e3 * dt / args.steps,
@@ -256,10 +293,7 @@ def main():
 
     tune = fixed
     if args.tune:
-        e = make_engine(S, w, precision, timing=False)
-        tune = e.autotune(precision)
-        e.close()
-        log(f"autotuned launch shape (targets/thread, source groups, slices) = {tune}")
+        tune = tuned_shape(S, w, precision)
 
     # mixed engine, timed region with HIP events around every Coulomb launch
     e, dt, st, status = run_engine(S, w, "mixed", args.steps, args.warmup, timing=True, tune=tune)
@@ -281,7 +315,8 @@ def main():
 
     # plain FP64 GPU engine on the same workload
     k64 = args.fp64_steps or max(2, args.steps // 4)
-    e64, dt64, st64, _ = run_engine(S, w, "fp64", k64, 1, timing=True, tune=tune)
+    tune64 = fixed if fixed is not None else (tuned_shape(S, w, "fp64") if args.tune else None)
+    e64, dt64, st64, _ = run_engine(S, w, "fp64", k64, 1, timing=True, tune=tune64)
     fp64 = {
         "pair_interactions_per_s": st64["pair_interactions"] / dt64,
         "ion_steps_per_s": st64["ion_steps"] / dt64,
@@ -295,7 +330,8 @@ def main():
     achieved = FLOP_PER_PAIR * n * (n - 1) / (coul_ms * 1e-3) / 1e12
     roofline = {
         "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-        "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+        "frac": achieved / PEAK_FP32_TFLOPS,
+        "traffic": MEASURED_TRAFFIC.get((info["name"].split(":")[0], n, tuple(tune or ()))),
         "kernel": "coulomb_tiled<mixed>", "flop_per_pair": FLOP_PER_PAIR,
         "pairs_per_launch": float(n) * (n - 1), "ms_per_launch": coul_ms,
         "fp64_kernel_frac_of_fp64_peak": FLOP_PER_PAIR * n * (n - 1)
@@ -325,7 +361,8 @@ def main():
         "data": "synthetic (seeded Gaussian / Maxwell-Boltzmann cloud)",
         "config": {"workload": w.name, "n_ions": n, "precision": "mixed", "rtol": w.rtol,
                    "atol": w.atol, "time_mode": "stage", "step": "one DP5 attempt",
-                   "launch_shape": tune, "device": info["name"], **w.notes},
+                   "launch_shape": tune, "fp64_launch_shape": tune64, "device": info["name"],
+                   **w.notes},
         "ion_steps_per_s": ion_steps,
         "accepted_steps": st["steps"],
         "attempts": st["attempts"],

@@ -27,6 +27,10 @@
 #ifndef SIMBUCA_HIP_H
 #define SIMBUCA_HIP_H
 
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -54,6 +58,29 @@ void cunbody1_force_end(double xj[][3], double mj[], double xi[][3], double eps2
  * whose backend precision is fixed inside a closed binary): SBK_FP64
  * (default) or SBK_MIXED.  Returns the previous setting. */
 int sbk_cunbody_set_precision(int precision);
+
+/* ------------------------------------------------------------------------
+ * Export set 1b -- reference NBODY interface, as its caller declares it
+ * (bin/nbody.cpp:12-18; the reference's own kernel file defines different,
+ * incompatible signatures at bin/nbody_kernel.cu:116,326).
+ *
+ * Positions go in as FP32 {x, y, z, w} (w ignored), accelerations come back
+ * as FP32 {x, y, z}: acc_i = sum_j (x_j - x_i) (|x_j - x_i|^2 + 1e-7)^{-3/2}
+ * (bin/nbody_kernel.cu:19,182-214; no charges, no ke).  The caller scales
+ * by -coulomb_scale e^2 ke / m_i (bin/force.cpp:97-104).  Work is enqueued
+ * on stream 0, so an event the caller records on stream 0 after
+ * simulation_Async completes with it (bin/nbody.cpp:243-245,265-268).
+ * p, q, ntiles are the reference's launch-table hints and are ignored.
+ * ---------------------------------------------------------------------- */
+typedef struct sbk_float4 { float x, y, z, w; } sbk_float4;   /* layout of float4 */
+typedef struct sbk_float3 { float x, y, z; } sbk_float3;      /* layout of float3 */
+void  initialize(int n_max, bool withCharge);
+float finalize(void);
+void  simulation_Async(sbk_float4 *h_p, sbk_float3 *res, int nbody, int p, int q, int ntiles);
+void  simulation_Async_begin(sbk_float4 *h_p, sbk_float3 *res, int nbody, int p, int q,
+                             int ntiles);
+void  simulation_Async_end(sbk_float4 *h_p, sbk_float3 *res, int nbody, int p, int q,
+                           int ntiles);
 
 /* ------------------------------------------------------------------------
  * Export set 2 -- device-resident engine.
@@ -169,7 +196,7 @@ int  sbk_eval_force(sbk_engine *e, int cloud, double *derivs);
  * computed with `precision`: out[n][3]. */
 int  sbk_eval_coulomb(sbk_engine *e, int cloud, int precision, double *out);
 
-/* Coulomb launch shape.  targets_per_thread in {1,2,4}; source_groups in
+/* Coulomb launch shape.  targets_per_thread in {1,2,4,8}; source_groups in
  * {1,4}: wave groups of a 256-thread work-group that share one set of
  * targets and split each source tile; slices = number of source slices per
  * target block (0 = automatic).  Applies to SBK_FP64 and SBK_MIXED*;
@@ -178,12 +205,20 @@ int  sbk_set_launch(sbk_engine *e, int targets_per_thread, int source_groups, in
 /* Times `reps` launches of the Coulomb kernel of the current state's
  * positions for every cloud; ms_per_launch receives the mean. */
 int  sbk_time_coulomb(sbk_engine *e, int precision, int reps, double *ms_per_launch);
-/* Tries the launch shapes for this engine's clouds and keeps the fastest;
- * returns the chosen shape.  One line per shape tried ("device ions clouds
- * precision targets_per_thread source_groups slices ms") is appended to
- * `table_path` if non-null. */
+/* Tries the launch shapes for this engine's clouds and keeps the fastest,
+ * timing each Coulomb launch together with one pass that reads its slice
+ * partials back (as every stage does); returns the chosen shape.  One line
+ * per shape tried ("device ions clouds precision targets_per_thread
+ * source_groups slices ms") is appended to `table_path` if non-null. */
 int  sbk_autotune(sbk_engine *e, int precision, int *targets_per_thread, int *source_groups,
                   int *slices, const char *table_path);
+
+/* Applies the fastest shape recorded in an sbk_autotune table for this
+ * device, ion count, cloud count and precision, without timing anything:
+ * the counterpart of the reference's per-GPU launch table read at start-up
+ * (bin/nbody.cpp:194-203,275-321).  Returns SBK_ESTATE when the table has
+ * no matching line (the automatic shape stays in use). */
+int  sbk_load_tuning(sbk_engine *e, int precision, const char *table_path);
 
 const char *sbk_last_error(void);
 /* Device name, compute-unit count and clock (MHz) of the current device. */

@@ -103,11 +103,12 @@ _lib = None
 # every symbol include/simbuca_hip.h declares
 EXPORTS = [
     "cunbody1_force", "cunbody1_force_begin", "cunbody1_force_end", "sbk_cunbody_set_precision",
+    "initialize", "finalize", "simulation_Async", "simulation_Async_begin", "simulation_Async_end",
     "sbk_default_config", "sbk_default_fields", "sbk_create", "sbk_destroy", "sbk_add_cloud",
     "sbk_set_fields", "sbk_set_step", "sbk_advance", "sbk_step", "sbk_attempts",
     "sbk_synchronize", "sbk_get_state", "sbk_get_status", "sbk_get_stats", "sbk_reset_stats",
     "sbk_eval_force", "sbk_eval_coulomb", "sbk_set_launch", "sbk_time_coulomb", "sbk_autotune",
-    "sbk_last_error", "sbk_device_info",
+    "sbk_load_tuning", "sbk_last_error", "sbk_device_info",
 ]
 
 
@@ -130,6 +131,11 @@ def lib():
         "cunbody1_force_begin": (None, [dp, dp, dp, c_dbl, dp, c_int, c_int]),
         "cunbody1_force_end": (None, [dp, dp, dp, c_dbl, dp, c_int, c_int]),
         "sbk_cunbody_set_precision": (c_int, [c_int]),
+        "initialize": (None, [c_int, ctypes.c_bool]),
+        "finalize": (ctypes.c_float, []),
+        "simulation_Async": (None, [vp, vp, c_int, c_int, c_int, c_int]),
+        "simulation_Async_begin": (None, [vp, vp, c_int, c_int, c_int, c_int]),
+        "simulation_Async_end": (None, [vp, vp, c_int, c_int, c_int, c_int]),
         "sbk_default_config": (None, [ctypes.POINTER(Config)]),
         "sbk_default_fields": (None, [ctypes.POINTER(Fields)]),
         "sbk_create": (c_int, [ctypes.POINTER(Config), ctypes.POINTER(vp)]),
@@ -150,6 +156,7 @@ def lib():
         "sbk_set_launch": (c_int, [vp, c_int, c_int, c_int]),
         "sbk_time_coulomb": (c_int, [vp, c_int, c_int, dp]),
         "sbk_autotune": (c_int, [vp, c_int, ip, ip, ip, ctypes.c_char_p]),
+        "sbk_load_tuning": (c_int, [vp, c_int, ctypes.c_char_p]),
         "sbk_last_error": (ctypes.c_char_p, []),
         "sbk_device_info": (c_int, [ctypes.c_char_p, c_int, ip, ip]),
     }
@@ -305,12 +312,41 @@ class Engine:
         _check(lib().sbk_time_coulomb(self._h, p, int(reps), ctypes.byref(ms)))
         return ms.value
 
+    def load_tuning(self, table_path, precision="mixed"):
+        """Applies the best shape recorded in an autotune table; False if none matches."""
+        p = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        rc = lib().sbk_load_tuning(self._h, p, table_path.encode())
+        if rc == -3:
+            return False
+        _check(rc)
+        return True
+
     def autotune(self, precision="mixed", table_path=None):
         p = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         t = ctypes.c_int(); q = ctypes.c_int(); s = ctypes.c_int()
         _check(lib().sbk_autotune(self._h, p, ctypes.byref(t), ctypes.byref(q), ctypes.byref(s),
                                   table_path.encode() if table_path else None))
         return t.value, q.value, s.value
+
+
+def nbody_accel(pos):
+    """The reference's NBODY call sequence (bin/nbody.cpp:24-76,216-271):
+    initialize, simulation_Async on FP32 float4 positions, wait, finalize.
+    Returns the FP32 accelerations sum_j (x_j-x_i)(|r|^2+1e-7)^-1.5."""
+    import ctypes as C
+    L = lib()
+    p = np.zeros((len(pos), 4), dtype=np.float32)
+    p[:, :3] = np.asarray(pos, dtype=np.float32).reshape(-1, 3)
+    p[:, 3] = 1.0                                     # w = 1 "charge" (bin/nbody.cpp:121)
+    res = np.zeros((len(pos), 3), dtype=np.float32)
+    L.initialize(len(pos), False)
+    try:
+        L.simulation_Async(p.ctypes.data_as(C.c_void_p), res.ctypes.data_as(C.c_void_p),
+                           len(pos), 0, 0, 0)
+        L.simulation_Async_end(None, None, 0, 0, 0, 0)
+    finally:
+        L.finalize()
+    return res
 
 
 def cunbody1_force(xj, mj, xi, eps2, precision=None):

@@ -890,7 +890,8 @@ int sbk_eval_coulomb(sbk_engine *e, int cloud, int precision, double *out)
 int sbk_set_launch(sbk_engine *e, int tpt, int groups, int slices)
 {
     if (!e) return fail(SBK_EINVAL, "null engine");
-    if (tpt != 1 && tpt != 2 && tpt != 4) return fail(SBK_EINVAL, "targets_per_thread must be 1, 2 or 4");
+    if (tpt != 1 && tpt != 2 && tpt != 4 && tpt != 8)
+        return fail(SBK_EINVAL, "targets_per_thread must be 1, 2, 4 or 8");
     if (groups != 1 && groups != 4) return fail(SBK_EINVAL, "source_groups must be 1 or 4");
     if (slices < 0) return fail(SBK_EINVAL, "slices must be >= 0");
     if (tpt != e->tpt || groups != e->groups || slices != e->slices_req) {
@@ -902,6 +903,38 @@ int sbk_set_launch(sbk_engine *e, int tpt, int groups, int slices)
     return 0;
 }
 
+// Mean time of `reps` Coulomb launches on the current positions; with_sum
+// adds the pass that reads every slice partial back (what each stage kernel
+// pays), so launch shapes with many slices are charged for their partials.
+static int time_coulomb_pass(sbk_engine *e, int precision, int reps, bool with_sum, double *ms_out)
+{
+    ItemList *L = nullptr;
+    int rc = get_list(e, precision, false, &L);
+    if (rc) return rc;
+    sbk::CoulombArgs a = coulomb_args(e, *L, e->dev.P0, false);
+    if (precision == SBK_FP64) a.eps_d = 1e-40;
+    sbk::EngineDev d = e->dev;
+    d.slices = L->d_slices;
+    hipEvent_t t0, t1;
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    HIP_TRY(hipEventRecord(t0, e->stream));
+    for (int i = 0; i < reps; ++i) {
+        if (sbk::launch_coulomb(kp(precision), e->tpt, e->groups, a, e->stream))
+            return fail(SBK_EHIP, "Coulomb kernel launch failed");
+        if (with_sum && sbk::launch_eval_derivs(d, 0, e->stream))
+            return fail(SBK_EHIP, "partial-sum kernel launch failed");
+    }
+    HIP_TRY(hipEventRecord(t1, e->stream));
+    HIP_TRY(hipEventSynchronize(t1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    *ms_out = ms / reps;
+    return 0;
+}
+
 int sbk_time_coulomb(sbk_engine *e, int precision, int reps, double *ms_per_launch)
 {
     int rc = prepare(e);
@@ -909,26 +942,7 @@ int sbk_time_coulomb(sbk_engine *e, int precision, int reps, double *ms_per_laun
     if (reps <= 0 || !ms_per_launch) return fail(SBK_EINVAL, "reps must be > 0");
     if (precision < SBK_FP64_EXACT || precision > SBK_MIXED_KAHAN)
         return fail(SBK_EINVAL, "unknown precision %d", precision);
-    ItemList *L = nullptr;
-    rc = get_list(e, precision, false, &L);
-    if (rc) return rc;
-    sbk::CoulombArgs a = coulomb_args(e, *L, e->dev.P0, false);
-    if (precision == SBK_FP64) a.eps_d = 1e-40;
-    hipEvent_t t0, t1;
-    HIP_TRY(hipEventCreate(&t0));
-    HIP_TRY(hipEventCreate(&t1));
-    HIP_TRY(hipEventRecord(t0, e->stream));
-    for (int i = 0; i < reps; ++i)
-        if (sbk::launch_coulomb(kp(precision), e->tpt, e->groups, a, e->stream))
-            return fail(SBK_EHIP, "Coulomb kernel launch failed");
-    HIP_TRY(hipEventRecord(t1, e->stream));
-    HIP_TRY(hipEventSynchronize(t1));
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
-    *ms_per_launch = ms / reps;
-    return 0;
+    return time_coulomb_pass(e, precision, reps, false, ms_per_launch);
 }
 
 int sbk_autotune(sbk_engine *e, int precision, int *tpt_out, int *groups_out, int *slices_out,
@@ -967,9 +981,9 @@ int sbk_autotune(sbk_engine *e, int precision, int *tpt_out, int *groups_out, in
                 e->groups = q;
                 e->slices_req = slices;
                 double ms = 0.0;
-                rc = sbk_time_coulomb(e, precision, 1, &ms);   // warm-up and list build
+                rc = time_coulomb_pass(e, precision, 1, true, &ms);   // warm-up, list build
                 if (rc) break;
-                rc = sbk_time_coulomb(e, precision, 3, &ms);
+                rc = time_coulomb_pass(e, precision, 3, true, &ms);
                 if (rc) break;
                 if (fp)
                     std::fprintf(fp, "%s\t%d\t%zu\t%d\t%d\t%d\t%d\t%.6f\n", dev_name, e->ntot,
@@ -996,6 +1010,38 @@ int sbk_autotune(sbk_engine *e, int precision, int *tpt_out, int *groups_out, in
     if (groups_out) *groups_out = best_q;
     if (slices_out) *slices_out = best_s;
     return 0;
+}
+
+int sbk_load_tuning(sbk_engine *e, int precision, const char *table)
+{
+    int rc = prepare(e);
+    if (rc) return rc;
+    if (!table) return fail(SBK_EINVAL, "null table path");
+    FILE *fp = std::fopen(table, "r");
+    if (!fp) return fail(SBK_ESTATE, "no tuning table at %s", table);
+    char dev_name[128] = {0};
+    sbk_device_info(dev_name, sizeof(dev_name), nullptr, nullptr);
+    char line[512], dev[256];
+    int ions, clouds, prec, t, q, sl;
+    double ms, best = 1e300;
+    int bt = 0, bq = 0, bs = 0;
+    while (std::fgets(line, sizeof(line), fp)) {
+        if (std::sscanf(line, "%255s %d %d %d %d %d %d %lf", dev, &ions, &clouds, &prec, &t, &q,
+                        &sl, &ms) != 8)
+            continue;
+        if (std::strcmp(dev, dev_name) || ions != e->ntot || clouds != (int)e->clouds.size() ||
+            prec != precision)
+            continue;
+        if (ms < best) {
+            best = ms;
+            bt = t;
+            bq = q;
+            bs = sl;
+        }
+    }
+    std::fclose(fp);
+    if (best == 1e300) return fail(SBK_ESTATE, "no matching line in %s", table);
+    return sbk_set_launch(e, bt, bq, bs);
 }
 
 }  // extern "C"

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
             lz[tid] = a.sz[t0 + tid];
             if (MODE == KP_FP64)
                 lqd[tid] = a.sqd[t0 + tid];
-            else
+            else if (!UNIT)
                 lqf[tid] = a.sqf[t0 + tid];
         }
         __syncthreads();
@@ -584,6 +584,7 @@ static const void *coulomb_fn(int precision, int tpt, int unit, int q)
     case 1: return tiled_fn<1>(precision, unit, q);
     case 2: return tiled_fn<2>(precision, unit, q);
     case 4: return tiled_fn<4>(precision, unit, q);
+    case 8: return tiled_fn<8>(precision, unit, q);
     default: return nullptr;
     }
 }

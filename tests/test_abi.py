@@ -17,7 +17,7 @@ from conftest import ROOT
 def _declared_symbols():
     src = open(os.path.join(ROOT, "include", "simbuca_hip.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    names = re.findall(r"\b((?:cunbody1|sbk)_\w+)\s*\(", src)
+    names = re.findall(r"^\s*(?:void|int|float|const char)\s*\*?\s*(\w+)\s*\(", src, flags=re.M)
     return sorted(set(names))
 
 

@@ -126,6 +126,24 @@ def test_launch_shapes_agree(rng, tpt, groups, slices):
     assert _rows_error(mixed, exact) < 1e-6
 
 
+def test_autotune_table_round_trip(rng, tmp_path):
+    amu, m, ch, pos, vel = _rand_cloud(rng, 2000, sigma=1e-3)
+    sf, _ = _both_fields(trap=0, coulomb=1)
+    table = str(tmp_path / "tune.tsv")
+    with S.Engine(precision="mixed") as e:
+        e.add_cloud(pos, vel, m, ch, sf)
+        shape = e.autotune("mixed", table)
+        ref = e.eval_coulomb(0, "mixed")
+    lines = open(table).read().strip().splitlines()
+    assert len(lines) == 3 * 2 * 6
+    with S.Engine(precision="mixed") as e:
+        e.add_cloud(pos, vel, m, ch, sf)
+        assert e.load_tuning(table, "mixed")
+        assert not e.load_tuning(table, "fp64")        # no FP64 lines in the table
+        again = e.eval_coulomb(0, "mixed")
+    assert np.array_equal(again, ref)                  # same shape -> same sums
+
+
 def test_multi_cloud_equals_single_clouds(rng):
     sizes = [64, 1, 300, 0, 1029]
     clouds = [_rand_cloud(rng, n) for n in sizes]
@@ -331,3 +349,24 @@ def test_cunbody_abi_fp64_and_mixed(rng):
     ref1 = O.cunbody_force(xj, ones, xi, 1e-20)
     assert _rows_error(S.cunbody1_force(xj, ones, xi, 1e-20, precision="mixed"), ref1) < 1e-6
     assert _rows_error(S.cunbody1_force(xj, ones, xi, 1e-20, precision="mixed_kahan"), ref1) < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# NBODY ABI (initialize / simulation_Async / finalize)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n", [1, 64, 1000, 5003])
+def test_nbody_abi(rng, n):
+    pos32 = rng.normal(0, 1e-3, (n, 3)).astype(np.float32)
+    acc = S.nbody_accel(pos32)
+    x = pos32.astype(np.float64)
+    ref = np.zeros((n, 3))
+    for i in range(n):
+        d = x - x[i]
+        r2 = (d * d).sum(1) + float(np.float32(1e-7))
+        ref[i] = (d / r2[:, None] ** 1.5).sum(0)
+    if n == 1:
+        assert np.all(acc == 0)
+        return
+    err = np.linalg.norm(acc - ref) / np.linalg.norm(ref)
+    assert err < 1e-6, err
