@@ -154,6 +154,66 @@ def force_error(S, w, e_mixed):
     }
 
 
+def cloud_invariants(c, pos, vel, B, cpen):
+    """Total energy (kinetic + trap + Coulomb) and canonical angular momentum
+    L_z = sum m (x v_y - y v_x) + q e B (x^2 + y^2)/2 of one cloud."""
+    from simbuca_amd import EL_CHARGE, KE
+    q = c.charge.astype(np.float64) * EL_CHARGE
+    kin = 0.5 * (c.mass[:, None] * vel * vel).sum()
+    trap = (q * cpen * (pos[:, 2] ** 2 - 0.5 * (pos[:, 0] ** 2 + pos[:, 1] ** 2))).sum()
+    coul = 0.0
+    for i in range(0, c.n, 512):
+        d = pos[i:i + 512, None, :] - pos[None, :, :]
+        r = np.sqrt((d * d).sum(-1))
+        idx = np.arange(i, min(i + 512, c.n))
+        r[idx - i, idx] = np.inf
+        coul += 0.5 * KE * (q[i:i + 512, None] * q[None, :] / r).sum()
+    lz = (c.mass * (pos[:, 0] * vel[:, 1] - pos[:, 1] * vel[:, 0])).sum() + \
+        (q * B * (pos[:, 0] ** 2 + pos[:, 1] ** 2) / 2).sum()
+    return kin + trap + coul, lz
+
+
+def invariant_drift(S):
+    """Energy and canonical L_z drift over the whole 100 us of the 500-ion
+    conservative Penning cloud (B, quadrupole, Coulomb; no drive, no friction)."""
+    from simbuca_amd import synth
+    w = synth.workload(1)
+    c = w.clouds[0]
+    B, V0, d0 = c.fields.Bz, c.fields.V0, c.fields.d0
+    cpen = V0 / (2 * d0 * d0)
+    e0, l0 = cloud_invariants(c, c.pos, c.vel, B, cpen)
+    out = {"workload": w.name, "t_sim": w.t_sim}
+    for p in ("mixed", "fp64"):
+        e = make_engine(S, w, p, timing=False)
+        e.advance(w.t_sim)
+        pos, vel = e.state(0)
+        st = e.status(0)
+        e.close()
+        e1, l1 = cloud_invariants(c, pos, vel, B, cpen)
+        out[p] = {"steps": st["steps"], "rel_energy_drift": (e1 - e0) / abs(e0),
+                  "rel_Lz_drift": (l1 - l0) / abs(l0)}
+    return out
+
+
+def trajectory_deviation(S, w, tune, tune64, steps=100):
+    """Max position difference, mixed vs plain-FP64 engine, after `steps`
+    fixed 1 ns steps from the same state, relative to the cloud's rms size."""
+    c = w.clouds[0]
+    res = {}
+    for p, sh in (("mixed", tune), ("fp64", tune64)):
+        e = S.Engine(precision=p, time_mode=w.time_mode, adaptive=False, h0=1e-9)
+        e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
+        if sh is not None:
+            e.set_launch(*sh)
+        e.attempts(steps)
+        e.synchronize()
+        res[p] = e.state(0)[0]
+        e.close()
+    dev = np.abs(res["mixed"] - res["fp64"]).max()
+    return {"steps": steps, "h": 1e-9, "max_abs_m": float(dev),
+            "relative_to_rms_size": float(dev / np.sqrt((res["fp64"] ** 2).mean()))}
+
+
 def cpu_baseline(w):
     """Reference CPU force() on the full 65,536-ion input, one call, 1 core."""
     import oracle
@@ -327,6 +387,11 @@ def main():
     e64.close()
     log(f"fp64: {fp64}")
 
+    traj = trajectory_deviation(S, w, tune, tune64)
+    log(f"trajectory deviation {traj}")
+    inv = invariant_drift(S)
+    log(f"invariants {inv}")
+
     achieved = FLOP_PER_PAIR * n * (n - 1) / (coul_ms * 1e-3) / 1e12
     roofline = {
         "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
@@ -370,6 +435,8 @@ def main():
         "fp64": fp64,
         "mixed_vs_fp64_ion_steps": ion_steps / fp64["ion_steps_per_s"] if fp64["ion_steps_per_s"] else None,
         "force_error": ferr,
+        "trajectory_deviation": traj,
+        "invariant_drift": inv,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

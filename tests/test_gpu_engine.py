@@ -284,6 +284,52 @@ def test_ensemble_clouds_have_independent_controllers(rng):
         assert np.allclose(gv, ov, rtol=1e-10, atol=1e-10 * np.abs(ov).max())
 
 
+def test_trap_ramp_on_device_matches_oracle(rng):
+    # the Vrf/Vdc ramp of move_particles (bin/ion_fly.cpp:235-239), evaluated on
+    # the device from the cloud's lifetime once per step
+    amu, m, ch, pos, vel = _rand_cloud(rng, 40, species=(133.0,), charges=(1,))
+    kw = dict(trap=1, Vrf=50.0, Vdc=5.0, kappa=0.001, r0=0.04, z0=0.04, freq_rf=5e6, coulomb=1,
+              ramp=1, ramp_t0=0.0, ramp_t1=1e-6, newVrf=20.0, newVdc=9.0)
+    sf, of = _both_fields(**kw)
+    with S.Engine(precision="fp64_exact", adaptive=False, h0=1e-8, use_graph=True) as e:
+        e.add_cloud(pos, vel, m, ch, sf)
+        e.advance(3e-7)
+        gp, gv = e.state(0)
+        st = e.status(0)
+    c = O.Cloud(pos, vel, m, ch, of, h0=1e-8, adaptive=False)
+    k, _ = c.advance(3e-7)
+    op, ov, sc = c.state()
+    assert st["steps"] == k >= 30
+    assert np.allclose(gp, op, rtol=1e-12, atol=1e-12 * np.abs(op).max())
+    assert np.allclose(gv, ov, rtol=1e-11, atol=1e-11 * np.abs(ov).max())
+
+
+def test_set_step_and_set_fields_between_runs(rng):
+    amu, m, ch, pos, vel = _rand_cloud(rng, 64, species=(133.0,), charges=(1,))
+    kw = dict(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0)
+    sf, of = _both_fields(**kw)
+    with S.Engine(precision="fp64_exact", adaptive=False, h0=1e-9, use_graph=True) as e:
+        e.add_cloud(pos, vel, m, ch, sf)
+        e.attempts(5)
+        e.set_step(0, 2e-9)
+        e.set_fields(0, S.default_fields(**dict(kw, delta=500.0)))
+        e.attempts(5)
+        e.synchronize()
+        gp, gv = e.state(0)
+        st = e.status(0)
+    c = O.Cloud(pos, vel, m, ch, of, h0=1e-9, adaptive=False)
+    for _ in range(5):
+        c.step()
+    of2 = O.fields(**dict(kw, delta=500.0))
+    p1, v1, sc1 = c.state()
+    c2 = O.Cloud(p1, v1, m, ch, of2, lifetime=sc1["lifetime"], h0=2e-9, adaptive=False)
+    for _ in range(5):
+        c2.step()
+    op, ov, sc = c2.state()
+    assert st["steps"] == 10 and st["lifetime"] == pytest.approx(sc["lifetime"], rel=1e-15)
+    assert np.array_equal(gp, op) and np.array_equal(gv, ov)
+
+
 def test_mixed_short_horizon_deviation(rng):
     # 100 fixed steps: mixed vs FP64 engine trajectories stay close
     w = synth.workload(2, n_override=4096)
