@@ -147,12 +147,13 @@ typedef struct sbk_cloud_status {
     long long steps, attempts;
     int    n;
     int    slices;      /* source slices of the Coulomb work split */
+    long long coulomb_evals;  /* Coulomb evaluations done (7 or 6 per attempt) */
 } sbk_cloud_status;
 
 typedef struct sbk_stats {
     double    coulomb_ms;        /* summed HIP-event time of Coulomb launches (timing=1) */
     long long coulomb_launches;  /* launches counted in coulomb_ms */
-    double    pair_interactions; /* sum over clouds of attempts*7*n(n-1) */
+    double    pair_interactions; /* sum over clouds of Coulomb evaluations * n(n-1) */
     long long attempts;          /* sum over clouds */
     long long steps;             /* accepted steps, summed over clouds */
     double    ion_steps;         /* sum over clouds of steps*n */
@@ -176,7 +177,13 @@ int  sbk_set_step(sbk_engine *e, int cloud, double h);
 
 /* Runs attempts until every cloud's lifetime reaches lifetime+duration
  * (the while-loop of move_particles, bin/ion_fly.cpp:83-101), or
- * max_attempts attempts were launched. */
+ * max_attempts attempts were launched.
+ * An attempt evaluates the Coulomb sum 7 times, as the reference does, except
+ * that the stage-1 derivative is reused when it is provably identical: after
+ * a rejected attempt (same y_n and t_n), and after an accepted one when the
+ * stage-7 derivative was taken at the new state and time (stage-time mode,
+ * or no time-dependent field; never with the trap ramp).  Results are
+ * bit-identical either way; those attempts cost 6 evaluations. */
 int  sbk_advance(sbk_engine *e, double duration, long long max_attempts);
 /* One accepted step for every cloud (step(), bin/ode.cpp:459-494). */
 int  sbk_step(sbk_engine *e);

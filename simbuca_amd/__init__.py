@@ -81,6 +81,7 @@ class CloudStatus(ctypes.Structure):
         ("error", ctypes.c_int),
         ("steps", ctypes.c_longlong), ("attempts", ctypes.c_longlong),
         ("n", ctypes.c_int), ("slices", ctypes.c_int),
+        ("coulomb_evals", ctypes.c_longlong),
     ]
 
     def as_dict(self):

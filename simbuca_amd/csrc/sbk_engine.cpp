@@ -110,7 +110,7 @@ struct sbk_engine {
     size_t ev_used = 0;
     double coulomb_ms = 0.0;
     long long coulomb_launches = 0;
-    std::vector<long long> attempts_at_reset, steps_at_reset;
+    std::vector<long long> attempts_at_reset, steps_at_reset, evals_at_reset;
 };
 
 namespace {
@@ -262,6 +262,13 @@ sbk::CloudParams cloud_params(const sbk_engine *e, size_t ci)
     sbk::CloudParams p = make_params(c.f, c.off, c.n);
     p.blk0 = e->blk0[ci];
     p.nblk = e->nblk[ci];
+    // Reusing an accepted attempt's stage-7 derivative as the next stage 1 is
+    // exact when stage 7 sees the new step's state and time: y7 is bit-equal
+    // to y_{n+1}, and with stage times t + c7 h = t + h = t_{n+1}.  With the
+    // reference's frozen time it is exact only if no term depends on time; the
+    // trap ramp is evaluated at the step's start time, so it never is.
+    const bool timed = (c.f.trap && c.f.Vrf != 0.0) || c.f.Ed != 0.0 || c.f.Uq != 0.0;
+    p.fsal = !c.f.ramp && (e->cfg.time_mode == SBK_TIME_STAGE || !timed);
     return p;
 }
 
@@ -398,6 +405,7 @@ int finalize(sbk_engine *e)
         for (int q : c.charge) e->unit_charges = e->unit_charges && q == 1;
     e->attempts_at_reset.assign(e->clouds.size(), 0);
     e->steps_at_reset.assign(e->clouds.size(), 0);
+    e->evals_at_reset.assign(e->clouds.size(), 0);
     e->finalized = true;
     HIP_TRY(hipStreamSynchronize(e->stream));
     return 0;
@@ -474,6 +482,7 @@ int enqueue_attempt(sbk_engine *e)
     for (int s = 1; s <= 7; ++s) {
         if (any_coulomb && L->nitems > 0) {
             sbk::CoulombArgs a = coulomb_args(e, *L, s == 1 ? d.P0 : d.PS, true);
+            a.skip_valid_k1 = s == 1;
             rc = launch_coulomb_timed(e, e->cfg.precision, a);
             if (rc) return rc;
         }
@@ -663,6 +672,12 @@ int sbk_set_fields(sbk_engine *e, int cloud, const sbk_fields *f)
     HIP_TRY(hipSetDevice(e->device));
     sbk::CloudParams p = cloud_params(e, cloud);
     HIP_TRY(hipMemcpyAsync(e->d_params + cloud, &p, sizeof(p), hipMemcpyHostToDevice, e->stream));
+    // the stored stage-1 derivative was computed with the old fields
+    int rc = read_states(e);
+    if (rc) return rc;
+    sbk::CloudState st = e->h_state[cloud];
+    st.k1_valid = 0;
+    HIP_TRY(hipMemcpyAsync(e->d_state + cloud, &st, sizeof(st), hipMemcpyHostToDevice, e->stream));
     if (scale_changed && c.n > 0) {
         std::vector<double> qm(c.n), kc(c.n), qd(c.n);
         std::vector<float> qf(c.n);
@@ -802,6 +817,7 @@ int sbk_get_status(sbk_engine *e, int cloud, sbk_cloud_status *st)
     st->steps = s.steps;
     st->attempts = s.attempts;
     st->n = e->clouds[cloud].n;
+    st->coulomb_evals = s.evals;
     ItemList *L = nullptr;
     st->slices = (get_list(e, e->cfg.precision, true, &L) == 0) ? L->slices[cloud] : 0;
     return 0;
@@ -824,7 +840,8 @@ int sbk_get_stats(sbk_engine *e, sbk_stats *out)
         out->attempts += att;
         out->steps += stp;
         out->ion_steps += (double)stp * n;
-        if (e->clouds[i].f.coulomb) out->pair_interactions += (double)att * 7.0 * n * (n - 1.0);
+        const long long ev = e->h_state[i].evals - e->evals_at_reset[i];
+        if (e->clouds[i].f.coulomb) out->pair_interactions += (double)ev * n * (n - 1.0);
     }
     return 0;
 }
@@ -838,6 +855,7 @@ int sbk_reset_stats(sbk_engine *e)
     for (size_t i = 0; i < e->clouds.size(); ++i) {
         e->attempts_at_reset[i] = e->h_state[i].attempts;
         e->steps_at_reset[i] = e->h_state[i].steps;
+        e->evals_at_reset[i] = e->h_state[i].evals;
     }
     e->coulomb_ms = 0.0;
     e->coulomb_launches = 0;

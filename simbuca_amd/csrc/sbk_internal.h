@@ -42,6 +42,9 @@ struct CloudParams {
     // layout
     int off, n, slices, pad;
     int blk0, nblk;           // this cloud's range in the ion-block table
+    int fsal;                 // an accepted attempt's stage-7 derivative is exactly the
+                              // next step's stage-1 derivative (same state and time)
+    int pad2;
 };
 
 struct IonBlock {
@@ -53,7 +56,8 @@ struct CloudState {
     int reject, active, accepted_last, error;
     long long steps, attempts;
     int budget;               // accepted steps still wanted (-1: unlimited)
-    int pad;
+    int k1_valid;             // KA[1] holds the derivative at (P0, V0, t)
+    long long evals;          // Coulomb evaluations done (6 or 7 per attempt)
 };
 
 // One Coulomb work item: targets [i0,i1) against sources [j0,j1) of one
@@ -75,6 +79,7 @@ struct CoulombArgs {
     int self_skip;                // FP64-exact: skip source index == target index
     int unit_weights;             // every source weight is exactly 1 (charge +1, mj = 1)
     const CloudState *state;      // nullptr: every item active
+    int skip_valid_k1;            // stage 1: skip clouds whose KA[1] is still valid
     float eps_f;
     double eps_d;
 };

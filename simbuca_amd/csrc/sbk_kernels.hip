@@ -40,7 +40,8 @@ __global__ __launch_bounds__(kBlock) void coulomb_fp64_exact(CoulombArgs a)
 {
     __shared__ double lx[kBlock], ly[kBlock], lz[kBlock], lq[kBlock];
     const CoulombItem it = a.items[xcd_item(blockIdx.x, gridDim.x)];
-    if (a.state && !a.state[it.cloud].active) return;
+    if (a.state && (!a.state[it.cloud].active || (a.skip_valid_k1 && a.state[it.cloud].k1_valid)))
+        return;
     const int tid = threadIdx.x;
     const int g = it.i0 + tid;
     const bool valid = g < it.i1;
@@ -106,7 +107,8 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
     __shared__ double lqd[(MODE == KP_FP64) ? kBlock : 1];
 
     const CoulombItem it = a.items[xcd_item(blockIdx.x, gridDim.x)];
-    if (a.state && !a.state[it.cloud].active) return;
+    if (a.state && (!a.state[it.cloud].active || (a.skip_valid_k1 && a.state[it.cloud].k1_valid)))
+        return;
     const int tid = threadIdx.x;
     const int lane = tid % L, grp = tid / L;
 
@@ -372,16 +374,22 @@ __global__ __launch_bounds__(256) void stage_kernel(EngineDev d)
     double *const *pos_s = (S == 1) ? d.P0 : d.PS;
     const double x = pos_s[0][g], y = pos_s[1][g], z = pos_s[2][g];
     const double vx = d.KV[S][0][g], vy = d.KV[S][1][g], vz = d.KV[S][2][g];
-    double Fx, Fy, Fz;
-    coulomb_total(d, p, c, g, Fx, Fy, Fz);
     double a3[3];
-    ion_accel(p, t, ts, d.qm[g], x, y, z, vx, vy, vz, Fx, Fy, Fz, a3[0], a3[1], a3[2]);
+    if (S == 1 && cs.k1_valid) {
+        // derivative at (y_n, t_n) kept from the previous attempt
+#pragma unroll
+        for (int k = 0; k < 3; ++k) a3[k] = d.KA[1][k][g];
+    } else {
+        double Fx, Fy, Fz;
+        coulomb_total(d, p, c, g, Fx, Fy, Fz);
+        ion_accel(p, t, ts, d.qm[g], x, y, z, vx, vy, vz, Fx, Fy, Fz, a3[0], a3[1], a3[2]);
+        if (valid) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) d.KA[S][k][g] = a3[k];
+        }
+    }
     const double v3[3] = {vx, vy, vz};
     const double s3[3] = {x, y, z};
-    if (valid) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) d.KA[S][k][g] = a3[k];
-    }
 
     if constexpr (S < 7) {
         if (!valid) return;
@@ -456,6 +464,7 @@ __global__ __launch_bounds__(256) void controller_kernel(EngineDev d)
     const double err = sqrt(red[0] / (6 * p.n));
     cs.last_err = err;
     cs.attempts++;
+    cs.evals += cs.k1_valid ? 6 : 7;
     int accept = 1;
     if (d.adaptive) {
         const double beta = 0.0, alpha = 0.2 - beta * 0.75, safe = 0.9, minscale = 0.2,
@@ -487,6 +496,10 @@ __global__ __launch_bounds__(256) void controller_kernel(EngineDev d)
         }
     }
     cs.accepted_last = accept;
+    // after a reject the next attempt restarts from the same (y_n, t_n): its
+    // stage-1 derivative is unchanged; after an accept it is the stage-7 one
+    // when that is exact (commit_kernel copies it)
+    cs.k1_valid = accept ? p.fsal : 1;
     if (accept) {
         cs.t = cs.t + cs.h;
         cs.h = cs.hnext;
@@ -504,10 +517,12 @@ __global__ __launch_bounds__(256) void commit_kernel(EngineDev d)
     if (!d.state[blk.cloud].accepted_last) return;
     const long long g = blk.start + threadIdx.x;
     if (g >= blk.end) return;
+    const bool fsal = d.params[blk.cloud].fsal;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         d.P0[k][g] = d.PS[k][g];
         d.V0[k][g] = d.KV[7][k][g];
+        if (fsal) d.KA[1][k][g] = d.KA[7][k][g];
     }
 }
 

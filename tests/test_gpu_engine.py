@@ -284,6 +284,40 @@ def test_ensemble_clouds_have_independent_controllers(rng):
         assert np.allclose(gv, ov, rtol=1e-10, atol=1e-10 * np.abs(ov).max())
 
 
+def test_stage1_reuse_counts_and_exactness(rng):
+    # Penning cloud in stage-time mode: every attempt after the first reuses
+    # its stage-1 derivative (6 Coulomb evaluations), with results identical
+    # to the oracle, which always evaluates 7 times
+    amu, m, ch, pos, vel = _rand_cloud(rng, 150, species=(133.0,), charges=(1,))
+    kw = dict(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0, Ed=20.0, fd=3e5)
+    sf, of = _both_fields(**kw)
+    with S.Engine(precision="fp64_exact", rtol=1e-9, atol=1e-9, time_mode=S.TIME_STAGE) as e:
+        e.add_cloud(pos, vel, m, ch, sf)
+        for _ in range(25):
+            e.step()
+        st = e.status(0)
+        stats = e.stats()
+        gp, _ = e.state(0)
+    assert st["coulomb_evals"] == 7 + 6 * (st["attempts"] - 1)
+    assert stats["pair_interactions"] == st["coulomb_evals"] * 150 * 149
+    c = O.Cloud(pos, vel, m, ch, of, h0=1e-9, rtol=1e-9, atol=1e-9, time_mode=O.TIME_STAGE)
+    for _ in range(25):
+        c.step()
+    op, _, sc = c.state()
+    assert st["attempts"] == sc["attempts"]
+    assert np.allclose(gp, op, rtol=1e-10, atol=1e-10 * np.abs(op).max())
+    # frozen time with an RF field: the stage-7 derivative is not the next
+    # stage 1 (different time), so only rejected attempts reuse it
+    kw = dict(trap=1, Vrf=50.0, Vdc=5.0, kappa=0.001, r0=0.04, z0=0.04, freq_rf=5e6, coulomb=1)
+    sf, _ = _both_fields(**kw)
+    with S.Engine(precision="fp64_exact", adaptive=False, h0=1e-9, time_mode=S.TIME_FROZEN) as e:
+        e.add_cloud(pos, vel, m, ch, sf)
+        for _ in range(5):
+            e.step()
+        st = e.status(0)
+    assert st["attempts"] == 5 and st["coulomb_evals"] == 35
+
+
 def test_trap_ramp_on_device_matches_oracle(rng):
     # the Vrf/Vdc ramp of move_particles (bin/ion_fly.cpp:235-239), evaluated on
     # the device from the cloud's lifetime once per step
@@ -356,7 +390,8 @@ def test_advance_reaches_t_end_and_counts(rng):
         stats = e.stats()
     assert st["lifetime"] >= 1.3e-6 and st["active"] == 0
     assert stats["steps"] == st["steps"]
-    assert stats["pair_interactions"] == st["attempts"] * 7 * 128 * 127
+    assert stats["pair_interactions"] == st["coulomb_evals"] * 128 * 127
+    assert 6 * st["attempts"] < st["coulomb_evals"] <= 7 * st["attempts"]
 
 
 def test_underflow_is_reported():
