@@ -63,7 +63,7 @@ def tuned_shape(S, w, precision):
         shape = e.autotune(precision)
     finally:
         e.close()
-    log(f"autotuned launch shape (targets/thread, source groups, slices) = {shape}")
+    log(f"autotuned launch shape (targets/thread, source groups, slices, local coords) = {shape}")
     return shape
 
 
@@ -74,13 +74,14 @@ def best_table_shape(table, device, w, precision):
     with open(table) as fh:
         for line in fh:
             f = line.split()
-            if len(f) != 8 or f[0] != device:
+            if len(f) not in (8, 9) or f[0] != device:
                 continue
             if int(f[1]) != n or int(f[2]) != len(w.clouds) or int(f[3]) != PRECISIONS[precision]:
                 continue
-            ms = float(f[7])
+            ms = float(f[-1])
+            local = int(f[7]) if len(f) == 9 else 0
             if best is None or ms < best[0]:
-                best = (ms, (int(f[4]), int(f[5]), int(f[6])))
+                best = (ms, (int(f[4]), int(f[5]), int(f[6]), local))
     return best[1] if best else None
 
 
@@ -124,14 +125,17 @@ def run_to_end(S, w, precision, tune=None):
 
 
 def force_error(S, w, e_mixed):
-    """Mixed Coulomb sum against the FP64-exact sum, both on the GPU at the
-    engine's current positions; the FP64-exact rows are checked against the
-    CPU oracle on a sample."""
+    """Mixed Coulomb sum of the engine against the FP64-exact sum at the
+    engine's current positions.  The FP64-exact sum comes from a second engine
+    that keeps the caller's ion order, so it sums in the oracle's order; its
+    rows are checked against the CPU oracle on a sample."""
     import oracle
     c = w.clouds[0]
-    pos, _ = e_mixed.state(0)
-    a64 = e_mixed.eval_coulomb(0, "fp64_exact")
-    amx = e_mixed.eval_coulomb(0, "mixed")
+    pos, vel = e_mixed.state(0)
+    with S.Engine(precision="fp64_exact", spatial_order=0) as ex:
+        ex.add_cloud(pos, vel, c.mass, c.charge, c.fields)
+        a64 = ex.eval_coulomb(0, "fp64_exact")
+    amx = e_mixed.eval_coulomb(0, e_mixed.precision)
     d = np.linalg.norm(amx - a64, axis=1)
     rel = d / np.linalg.norm(a64, axis=1)
     rows = np.random.Generator(np.random.PCG64(11)).choice(c.n, 64, replace=False).astype(np.int32)
@@ -314,8 +318,8 @@ def main():
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="skip the autotuner and use the automatic launch shape")
     ap.add_argument("--shape", default="",
-                    help="fixed Coulomb launch shape R,Q,S (targets/thread, source groups, "
-                         "slices); skips the autotuner")
+                    help="fixed Coulomb launch shape R,Q,S[,L] (targets/thread, source groups, "
+                         "slices, local coordinates -1/0/1); skips the autotuner")
     ap.add_argument("--n", type=int, default=0, help="override the ion count (testing)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the Coulomb kernels and a 1 GiB device copy (profiler runs)")
@@ -372,24 +376,27 @@ def main():
         f"({evals:.0f} evaluations in {st['coulomb_launches']} launches)")
     ferr = force_error(S, w, e)
     log(f"force error {ferr}")
+    e_cfg = {"spatial_order": e.config.spatial_order, "reorder_interval": e.config.reorder_interval}
     e.close()
 
     # mixed engine replayed from the captured hipGraph (no events)
-    eg, dtg, stg, _ = run_engine(S, w, "mixed", args.steps, 1, timing=False, tune=tune)
+    eg, dtg, stg, _ = run_engine(S, w, "mixed", args.steps, args.warmup, timing=False, tune=tune)
     graph = {"pair_interactions_per_s": stg["pair_interactions"] / dtg,
              "ion_steps_per_s": stg["ion_steps"] / dtg, "ms_per_step": 1e3 * dtg / args.steps}
     eg.close()
 
-    # plain FP64 GPU engine on the same workload
-    k64 = args.fp64_steps or max(2, args.steps // 4)
+    # plain FP64 GPU engine on the same workload and the same attempt window
+    # (same warm-up, same count), so both count the same accepted steps
+    k64 = args.fp64_steps or args.steps
     tune64 = fixed if fixed is not None else (tuned_shape(S, w, "fp64") if args.tune else None)
-    e64, dt64, st64, _ = run_engine(S, w, "fp64", k64, 1, timing=True, tune=tune64)
+    e64, dt64, st64, _ = run_engine(S, w, "fp64", k64, args.warmup, timing=True, tune=tune64)
     fp64 = {
         "pair_interactions_per_s": st64["pair_interactions"] / dt64,
         "ion_steps_per_s": st64["ion_steps"] / dt64,
         "ms_per_step": 1e3 * dt64 / k64,
         "coulomb_ms_per_eval": st64["coulomb_ms"] / max(1.0, st64["pair_interactions"] / (float(n) * (n - 1))),
         "steps": k64,
+        "accepted_steps": st64["steps"],
     }
     e64.close()
     log(f"fp64: {fp64}")
@@ -432,11 +439,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair math from f64 separations, f64 accumulation and DP5",
+        "dtype": "f32 pair math (separations from f64 positions, or f32 offsets for far sub-tiles), "
+                 "f64 accumulation and DP5",
         "data": "synthetic (seeded Gaussian / Maxwell-Boltzmann cloud)",
         "config": {"workload": w.name, "n_ions": n, "precision": "mixed", "rtol": w.rtol,
                    "atol": w.atol, "time_mode": "stage", "step": "one DP5 attempt",
                    "launch_shape": tune, "fp64_launch_shape": tune64, "device": info["name"],
+                   "spatial_order": e_cfg["spatial_order"], "reorder_interval": e_cfg["reorder_interval"],
                    **w.notes},
         "ion_steps_per_s": ion_steps,
         "accepted_steps": st["steps"],

@@ -138,6 +138,13 @@ typedef struct sbk_config {
     int    timing;      /* 1: time every Coulomb launch with HIP events */
     double rtol, atol;  /* the reference hard-codes 1e-7/1e-7 (bin/ode.cpp:515-516) */
     double h0;          /* initial step (the [ode] timestep of the reference) */
+    int    spatial_order;    /* mixed precisions: 1 keeps each cloud's ions in a
+                                spatial order on the device, so most Coulomb
+                                sub-tiles form separations from FP32 offsets
+                                (faster, same accuracy); results are returned
+                                in the caller's order either way */
+    int    reorder_interval; /* attempts between re-sorts of that order as the
+                                ions move (0: sort once, at the first run) */
 } sbk_config;
 
 typedef struct sbk_cloud_status {
@@ -190,6 +197,11 @@ int  sbk_step(sbk_engine *e);
 /* Enqueues `count` attempts for the active clouds without waiting. */
 int  sbk_attempts(sbk_engine *e, int count);
 int  sbk_synchronize(sbk_engine *e);
+/* Re-sorts the device order of every cloud by its current positions now
+ * (spatial_order engines; otherwise does nothing).  sbk_advance does this by
+ * itself every reorder_interval attempts.  No counterpart in the reference,
+ * whose ion order never matters for speed. */
+int  sbk_reorder(sbk_engine *e);
 
 int  sbk_get_state(sbk_engine *e, int cloud, double *pos, double *vel);
 int  sbk_get_status(sbk_engine *e, int cloud, sbk_cloud_status *st);
@@ -206,9 +218,14 @@ int  sbk_eval_coulomb(sbk_engine *e, int cloud, int precision, double *out);
 /* Coulomb launch shape.  targets_per_thread in {1,2,4,8}; source_groups in
  * {1,4}: wave groups of a 256-thread work-group that share one set of
  * targets and split each source tile; slices = number of source slices per
- * target block (0 = automatic).  Applies to SBK_FP64 and SBK_MIXED*;
- * SBK_FP64_EXACT always uses one thread per target and one slice. */
-int  sbk_set_launch(sbk_engine *e, int targets_per_thread, int source_groups, int slices);
+ * target block (0 = automatic); local_coords (spatial_order mixed engines):
+ * 1 lets far source sub-tiles form separations from FP32 offsets, 0 keeps
+ * FP64-formed separations everywhere, -1 (automatic) chooses 1 when the
+ * clouds' pair-weighted mean size is at least 32768 ions.  Applies to
+ * SBK_FP64 and SBK_MIXED*; SBK_FP64_EXACT always uses one thread per target
+ * and one slice. */
+int  sbk_set_launch(sbk_engine *e, int targets_per_thread, int source_groups, int slices,
+                    int local_coords);
 /* Times `reps` launches of the Coulomb kernel of the current state's
  * positions for every cloud; ms_per_launch receives the mean. */
 int  sbk_time_coulomb(sbk_engine *e, int precision, int reps, double *ms_per_launch);
@@ -216,16 +233,25 @@ int  sbk_time_coulomb(sbk_engine *e, int precision, int reps, double *ms_per_lau
  * timing each Coulomb launch together with one pass that reads its slice
  * partials back (as every stage does); returns the chosen shape.  One line
  * per shape tried ("device ions clouds precision targets_per_thread
- * source_groups slices ms") is appended to `table_path` if non-null. */
+ * source_groups slices local_coords ms") is appended to `table_path` if
+ * non-null. */
 int  sbk_autotune(sbk_engine *e, int precision, int *targets_per_thread, int *source_groups,
-                  int *slices, const char *table_path);
+                  int *slices, int *local_coords, const char *table_path);
 
 /* Applies the fastest shape recorded in an sbk_autotune table for this
- * device, ion count, cloud count and precision, without timing anything:
+ * device, ion count, cloud count and precision, without timing anything
+ * (lines without the local_coords column are read as local_coords 0):
  * the counterpart of the reference's per-GPU launch table read at start-up
  * (bin/nbody.cpp:194-203,275-321).  Returns SBK_ESTATE when the table has
  * no matching line (the automatic shape stays in use). */
 int  sbk_load_tuning(sbk_engine *e, int precision, const char *table_path);
+
+/* The device order a spatial_order engine gives one cloud: order[i] is the
+ * index (0..n-1, into the AoS [n][3] positions) of the ion placed in slot i.
+ * Host only; recursive bisection along the longest axis, with the first part
+ * a multiple of the largest power of two A <= 1024 with 2A <= part size, so
+ * aligned runs of 64..1024 ions are spatially compact. */
+int  sbk_spatial_order(int n, const double *pos, int *order);
 
 const char *sbk_last_error(void);
 /* Device name, compute-unit count and clock (MHz) of the current device. */

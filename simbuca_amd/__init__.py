@@ -70,6 +70,7 @@ class Config(ctypes.Structure):
         ("precision", ctypes.c_int), ("time_mode", ctypes.c_int), ("adaptive", ctypes.c_int),
         ("use_graph", ctypes.c_int), ("timing", ctypes.c_int),
         ("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("h0", ctypes.c_double),
+        ("spatial_order", ctypes.c_int), ("reorder_interval", ctypes.c_int),
     ]
 
 
@@ -107,9 +108,9 @@ EXPORTS = [
     "initialize", "finalize", "simulation_Async", "simulation_Async_begin", "simulation_Async_end",
     "sbk_default_config", "sbk_default_fields", "sbk_create", "sbk_destroy", "sbk_add_cloud",
     "sbk_set_fields", "sbk_set_step", "sbk_advance", "sbk_step", "sbk_attempts",
-    "sbk_synchronize", "sbk_get_state", "sbk_get_status", "sbk_get_stats", "sbk_reset_stats",
+    "sbk_synchronize", "sbk_reorder", "sbk_get_state", "sbk_get_status", "sbk_get_stats", "sbk_reset_stats",
     "sbk_eval_force", "sbk_eval_coulomb", "sbk_set_launch", "sbk_time_coulomb", "sbk_autotune",
-    "sbk_load_tuning", "sbk_last_error", "sbk_device_info",
+    "sbk_load_tuning", "sbk_spatial_order", "sbk_last_error", "sbk_device_info",
 ]
 
 
@@ -148,16 +149,18 @@ def lib():
         "sbk_step": (c_int, [vp]),
         "sbk_attempts": (c_int, [vp, c_int]),
         "sbk_synchronize": (c_int, [vp]),
+        "sbk_reorder": (c_int, [vp]),
         "sbk_get_state": (c_int, [vp, c_int, dp, dp]),
         "sbk_get_status": (c_int, [vp, c_int, ctypes.POINTER(CloudStatus)]),
         "sbk_get_stats": (c_int, [vp, ctypes.POINTER(Stats)]),
         "sbk_reset_stats": (c_int, [vp]),
         "sbk_eval_force": (c_int, [vp, c_int, dp]),
         "sbk_eval_coulomb": (c_int, [vp, c_int, c_int, dp]),
-        "sbk_set_launch": (c_int, [vp, c_int, c_int, c_int]),
+        "sbk_set_launch": (c_int, [vp, c_int, c_int, c_int, c_int]),
         "sbk_time_coulomb": (c_int, [vp, c_int, c_int, dp]),
-        "sbk_autotune": (c_int, [vp, c_int, ip, ip, ip, ctypes.c_char_p]),
+        "sbk_autotune": (c_int, [vp, c_int, ip, ip, ip, ip, ctypes.c_char_p]),
         "sbk_load_tuning": (c_int, [vp, c_int, ctypes.c_char_p]),
+        "sbk_spatial_order": (c_int, [c_int, dp, ip]),
         "sbk_last_error": (ctypes.c_char_p, []),
         "sbk_device_info": (c_int, [ctypes.c_char_p, c_int, ip, ip]),
     }
@@ -196,6 +199,15 @@ def default_fields(**kw) -> Fields:
     return f
 
 
+def spatial_order(pos):
+    """The order a spatial_order engine keeps a cloud's ions in on the device
+    (host computation, no GPU needed): slot i holds ion order[i]."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    order = np.empty(pos.shape[0], dtype=np.int32)
+    _check(lib().sbk_spatial_order(pos.shape[0], _dp(pos), _ip(order)))
+    return order
+
+
 def device_info():
     name = ctypes.create_string_buffer(128)
     cus = ctypes.c_int(); clk = ctypes.c_int()
@@ -207,7 +219,8 @@ class Engine:
     """Device-resident clouds advanced by Dormand-Prince 5(4) on the GPU."""
 
     def __init__(self, precision="mixed", time_mode=TIME_FROZEN, adaptive=True, use_graph=True,
-                 timing=False, rtol=1e-7, atol=1e-7, h0=1e-9):
+                 timing=False, rtol=1e-7, atol=1e-7, h0=1e-9, spatial_order=None,
+                 reorder_interval=None):
         L = lib()
         cfg = Config()
         L.sbk_default_config(ctypes.byref(cfg))
@@ -217,10 +230,15 @@ class Engine:
         cfg.use_graph = 1 if use_graph else 0
         cfg.timing = 1 if timing else 0
         cfg.rtol, cfg.atol, cfg.h0 = float(rtol), float(atol), float(h0)
+        if spatial_order is not None:
+            cfg.spatial_order = 1 if spatial_order else 0
+        if reorder_interval is not None:
+            cfg.reorder_interval = int(reorder_interval)
         h = ctypes.c_void_p()
         _check(L.sbk_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
         self.config = cfg
+        self.precision = next(k for k, v in PRECISIONS.items() if v == cfg.precision)
         self.sizes = []
 
     def close(self):
@@ -257,9 +275,9 @@ class Engine:
     def set_step(self, cloud, h):
         _check(lib().sbk_set_step(self._h, cloud, float(h)))
 
-    def set_launch(self, targets_per_thread=2, source_groups=4, slices=0):
+    def set_launch(self, targets_per_thread=4, source_groups=4, slices=0, local_coords=-1):
         _check(lib().sbk_set_launch(self._h, int(targets_per_thread), int(source_groups),
-                                    int(slices)))
+                                    int(slices), int(local_coords)))
 
     def advance(self, duration, max_attempts=1 << 62):
         _check(lib().sbk_advance(self._h, float(duration), int(max_attempts)))
@@ -276,6 +294,9 @@ class Engine:
 
     def synchronize(self):
         _check(lib().sbk_synchronize(self._h))
+
+    def reorder(self):
+        _check(lib().sbk_reorder(self._h))
 
     def state(self, cloud=0):
         n = self.sizes[cloud]
@@ -324,10 +345,10 @@ class Engine:
 
     def autotune(self, precision="mixed", table_path=None):
         p = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
-        t = ctypes.c_int(); q = ctypes.c_int(); s = ctypes.c_int()
+        t = ctypes.c_int(); q = ctypes.c_int(); s = ctypes.c_int(); lc = ctypes.c_int()
         _check(lib().sbk_autotune(self._h, p, ctypes.byref(t), ctypes.byref(q), ctypes.byref(s),
-                                  table_path.encode() if table_path else None))
-        return t.value, q.value, s.value
+                                  ctypes.byref(lc), table_path.encode() if table_path else None))
+        return t.value, q.value, s.value, lc.value
 
 
 def nbody_accel(pos):

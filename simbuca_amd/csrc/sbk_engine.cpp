@@ -17,6 +17,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <numeric>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -56,8 +57,9 @@ struct HostCloud {
     int n = 0, off = 0;
     sbk_fields f{};
     double lifetime = 0.0;
-    std::vector<double> pos, vel, mass;
+    std::vector<double> pos, vel, mass;   // in device (slot) order after finalize
     std::vector<int> charge;
+    std::vector<int> perm;                // slot -> caller's index (empty: identity)
 };
 
 struct ItemList {
@@ -93,13 +95,18 @@ struct sbk_engine {
     sbk::CloudState *h_state = nullptr;   // pinned mirror
     double *d_kc = nullptr, *d_qd = nullptr;
     bool unit_charges = false;   // every ion has charge +1
+    bool spatial = false;        // ions kept in a spatial order (mixed precisions)
+    int *d_src = nullptr;        // gather indices of a re-sort
+    long long attempts_at_order = 0;
+    long long reorders = 0;
     sbk::EngineDev dev{};
 
     // launch shape
     int tpt = 4;
     int groups = 4;
     int slices_req = 0;
-    std::map<std::tuple<int, int, int, int, int>, ItemList> lists;
+    int local_req = -1;          // -1: automatic
+    std::map<std::tuple<int, int, int, int, int, int>, ItemList> lists;
 
     // graph of one attempt
     hipGraph_t graph = nullptr;
@@ -153,6 +160,70 @@ int kp(int precision)
     return precision;  // SBK_* and sbk::KP_* share values
 }
 
+// Spatial order of one cloud: recursive bisection along the longest axis of
+// the bounding box, where the first part holds a multiple of the largest
+// power of two A <= 1024 with 2A <= len ions.  Every aligned run of
+// 64 * 2^k <= 1024 ions (the Coulomb kernel's target blocks and source
+// sub-tiles, which start at the cloud's first ion) is then one subtree, so
+// the ions of a block are close together.  A leaf (at most 64 ions) is
+// sorted by position, so the order depends only on the positions, not on
+// the order they came in: re-sorting a sorted cloud changes nothing.
+// pos is AoS [n][3].
+void bisect(const double *pos, int *idx, int len)
+{
+    if (len <= 64) {
+        std::sort(idx, idx + len, [pos](int a, int b) {
+            const double *p = pos + 3 * (size_t)a, *q = pos + 3 * (size_t)b;
+            if (p[0] != q[0]) return p[0] < q[0];
+            if (p[1] != q[1]) return p[1] < q[1];
+            if (p[2] != q[2]) return p[2] < q[2];
+            return a < b;
+        });
+        return;
+    }
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = 0; i < len; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const double v = pos[3 * (size_t)idx[i] + k];
+            mn[k] = std::min(mn[k], v);
+            mx[k] = std::max(mx[k], v);
+        }
+    int ax = 0;
+    for (int k = 1; k < 3; ++k)
+        if (mx[k] - mn[k] > mx[ax] - mn[ax]) ax = k;
+    int A = 1024;
+    while (A > 64 && 2 * A > len) A /= 2;
+    const int m = std::max(A, (int)std::lround(len / (2.0 * A)) * A);   // A <= m < len
+    std::nth_element(idx, idx + m, idx + len, [pos, ax](int a, int b) {
+        return pos[3 * (size_t)a + ax] < pos[3 * (size_t)b + ax];
+    });
+    bisect(pos, idx, m);
+    bisect(pos, idx + m, len - m);
+}
+
+std::vector<int> spatial_order(const double *pos, int n)
+{
+    std::vector<int> ord(n);
+    std::iota(ord.begin(), ord.end(), 0);
+    if (n > 0) bisect(pos, ord.data(), n);
+    return ord;
+}
+
+template <typename T>
+void apply_order(std::vector<T> &v, const std::vector<int> &ord, int width)
+{
+    std::vector<T> out(v.size());
+    for (size_t i = 0; i < ord.size(); ++i)
+        for (int k = 0; k < width; ++k) out[width * i + k] = v[width * (size_t)ord[i] + k];
+    v.swap(out);
+}
+
+// caller's index of the ion in slot i of a cloud
+inline int caller_index(const HostCloud &c, int i)
+{
+    return c.perm.empty() ? i : c.perm[i];
+}
+
 void destroy_graph(sbk_engine *e)
 {
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
@@ -171,6 +242,23 @@ void free_lists(sbk_engine *e)
     destroy_graph(e);
 }
 
+// Whether the mixed Coulomb kernel uses FP32 offsets for far sub-tiles.  It
+// pays off once a target block is a small part of its cloud (measured on
+// MI355X: 17-26% faster at 65,536-262,144 ions, slower at 10,000 and below,
+// where the per-work-group set-up dominates), hence the automatic rule on
+// the pair-weighted mean cloud size.
+int local_coords(const sbk_engine *e, int precision)
+{
+    if (!e->spatial || (precision != SBK_MIXED && precision != SBK_MIXED_KAHAN)) return 0;
+    if (e->local_req >= 0) return e->local_req;
+    double n2 = 0.0, n3 = 0.0;
+    for (auto &c : e->clouds) {
+        n2 += (double)c.n * c.n;
+        n3 += (double)c.n * c.n * c.n;
+    }
+    return n2 > 0.0 && n3 / n2 >= 32768.0 ? 1 : 0;
+}
+
 // Builds (or returns the cached) Coulomb work list for a precision and
 // launch shape.  `only_coulomb` keeps the clouds whose Coulomb switch is on.
 int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
@@ -179,7 +267,8 @@ int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
     const int tpt = exact ? 1 : e->tpt;
     const int q = exact ? 1 : e->groups;
     const int sreq = exact ? 1 : e->slices_req;
-    auto key = std::make_tuple(precision, tpt, q, sreq, only_coulomb ? 1 : 0);
+    auto key = std::make_tuple(precision, tpt, q, sreq, only_coulomb ? 1 : 0,
+                               local_coords(e, precision));
     auto it = e->lists.find(key);
     if (it != e->lists.end()) {
         *out = &it->second;
@@ -192,7 +281,9 @@ int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
     // automatic split: about eight rounds of resident work-groups; short
     // items keep the last, partially filled round (the tail) short
     const long long capacity =
-        (long long)sbk::coulomb_blocks_per_cu(precision, tpt, e->unit_charges ? 1 : 0, q) * e->cus;
+        (long long)sbk::coulomb_blocks_per_cu(precision, tpt, e->unit_charges ? 1 : 0, q,
+                                              local_coords(e, precision)) *
+        e->cus;
     const long long target = 8 * capacity;
     ItemList L;
     std::vector<sbk::CoulombItem> items;
@@ -310,6 +401,19 @@ int finalize(sbk_engine *e)
     }
     e->stride = std::max<long long>(64, ((long long)e->ntot + 63) / 64 * 64);
     const long long S = e->stride;
+    e->spatial = e->cfg.spatial_order &&
+                 (e->cfg.precision == SBK_MIXED || e->cfg.precision == SBK_MIXED_KAHAN);
+    if (e->spatial) {
+        for (auto &c : e->clouds) {
+            const std::vector<int> ord = spatial_order(c.pos.data(), c.n);
+            apply_order(c.pos, ord, 3);
+            apply_order(c.vel, ord, 3);
+            apply_order(c.mass, ord, 1);
+            apply_order(c.charge, ord, 1);
+            c.perm = ord;
+        }
+        HIP_TRY(hipMalloc(&e->d_src, sizeof(int) * S));
+    }
     HIP_TRY(hipMalloc(&e->d_f64, sizeof(double) * S * A_COUNT));
     HIP_TRY(hipMemsetAsync(e->d_f64, 0, sizeof(double) * S * A_COUNT, e->stream));
     HIP_TRY(hipMalloc(&e->d_qf, sizeof(float) * S));
@@ -450,6 +554,7 @@ int launch_coulomb_timed(sbk_engine *e, int precision, const sbk::CoulombArgs &a
     }
     sbk::CoulombArgs b = a;
     if (precision == SBK_FP64) b.eps_d = 1e-40;   // the guard of bin/force.cpp:136
+    b.local_coords = local_coords(e, precision);
     if (sbk::launch_coulomb(kp(precision), e->tpt, e->groups, b, e->stream))
         return fail(SBK_EHIP, "Coulomb kernel launch failed: %s",
                     hipGetErrorString(hipGetLastError()));
@@ -531,6 +636,68 @@ int read_states(sbk_engine *e)
     return harvest_events(e);
 }
 
+// Re-sorts every cloud into a spatial order of its current positions and
+// permutes the per-ion arrays that live from one attempt to the next
+// (positions, velocities, the stage-1 derivative and the ion constants).
+// The stage arrays are dead between attempts and serve as the gather buffer.
+int reorder(sbk_engine *e)
+{
+    if (!e->spatial) return 0;
+    const long long S = e->stride;
+    const int n = e->ntot;
+    std::vector<double> soa(3 * (size_t)S);
+    for (int k = 0; k < 3; ++k)
+        HIP_TRY(hipMemcpyAsync(soa.data() + k * S, e->dev.P0[k], sizeof(double) * n,
+                               hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    std::vector<int> src(n);
+    std::vector<double> aos;
+    for (auto &c : e->clouds) {
+        aos.resize(3 * (size_t)c.n);
+        for (int i = 0; i < c.n; ++i)
+            for (int k = 0; k < 3; ++k) aos[3 * (size_t)i + k] = soa[k * S + c.off + i];
+        const std::vector<int> ord = spatial_order(aos.data(), c.n);
+        std::vector<int> perm(c.n);
+        for (int i = 0; i < c.n; ++i) {
+            src[c.off + i] = c.off + ord[i];
+            perm[i] = caller_index(c, ord[i]);
+        }
+        apply_order(c.mass, ord, 1);
+        apply_order(c.charge, ord, 1);
+        c.perm.swap(perm);
+    }
+    HIP_TRY(hipMemcpyAsync(e->d_src, src.data(), sizeof(int) * n, hipMemcpyHostToDevice, e->stream));
+    double *base = e->d_f64;
+    double *tmp = base + A_PS * S;
+    double *arrays[12] = {e->dev.P0[0],    e->dev.P0[1],    e->dev.P0[2],    e->dev.V0[0],
+                          e->dev.V0[1],    e->dev.V0[2],    e->dev.KA[1][0], e->dev.KA[1][1],
+                          e->dev.KA[1][2], base + A_QM * S, base + A_KC * S, base + A_QD * S};
+    for (double *a : arrays) {
+        if (sbk::launch_gather(e->d_src, a, tmp, n, e->stream))
+            return fail(SBK_EHIP, "gather kernel launch failed");
+        HIP_TRY(hipMemcpyAsync(a, tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, e->stream));
+    }
+    float *tmpf = reinterpret_cast<float *>(tmp);
+    if (sbk::launch_gather_f32(e->d_src, e->d_qf, tmpf, n, e->stream))
+        return fail(SBK_EHIP, "gather kernel launch failed");
+    HIP_TRY(hipMemcpyAsync(e->d_qf, tmpf, sizeof(float) * n, hipMemcpyDeviceToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));   // src goes out of scope
+    e->reorders++;
+    return 0;
+}
+
+// Re-sorts when reorder_interval attempts have run since the last sort; the
+// per-cloud states must be fresh (read_states) and the stream idle.
+int maybe_reorder(sbk_engine *e)
+{
+    if (!e->spatial || e->cfg.reorder_interval <= 0) return 0;
+    long long att = 0;
+    for (size_t i = 0; i < e->clouds.size(); ++i) att += e->h_state[i].attempts;
+    if (att - e->attempts_at_order < e->cfg.reorder_interval) return 0;
+    e->attempts_at_order = att;
+    return reorder(e);
+}
+
 int prepare(sbk_engine *e)
 {
     if (!e) return fail(SBK_EINVAL, "null engine");
@@ -559,6 +726,8 @@ void sbk_default_config(sbk_config *c)
     c->rtol = 1e-7;   // bin/ode.cpp:515-516
     c->atol = 1e-7;
     c->h0 = 1e-9;     // bin/ode.cpp:509
+    c->spatial_order = 1;
+    c->reorder_interval = 256;
 }
 
 void sbk_default_fields(sbk_fields *f)
@@ -576,6 +745,14 @@ void sbk_default_fields(sbk_fields *f)
     f->coulomb = 0;
     f->coulomb_scale = 1.0;
     f->rq = 1.0;
+}
+
+int sbk_spatial_order(int n, const double *pos, int *order)
+{
+    if (n < 0 || (n > 0 && (!pos || !order))) return fail(SBK_EINVAL, "bad arguments (n=%d)", n);
+    const std::vector<int> ord = spatial_order(pos, n);
+    std::copy(ord.begin(), ord.end(), order);
+    return 0;
 }
 
 int sbk_device_info(char *name, int name_len, int *cus, int *clock_mhz)
@@ -630,6 +807,7 @@ int sbk_destroy(sbk_engine *e)
     (void)hipFree(e->d_cp);
     (void)hipFree(e->d_params);
     (void)hipFree(e->d_state);
+    (void)hipFree(e->d_src);
     if (e->h_state) (void)hipHostFree(e->h_state);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -769,9 +947,23 @@ int sbk_advance(sbk_engine *e, double duration, long long max_attempts)
         bool any = false;
         for (size_t i = 0; i < e->clouds.size(); ++i) any |= e->h_state[i].active != 0;
         if (!any) break;
+        rc = maybe_reorder(e);
+        if (rc) return rc;
         batch = std::min(batch * 2, 64);
     }
     return 0;
+}
+
+int sbk_reorder(sbk_engine *e)
+{
+    int rc = prepare(e);
+    if (rc) return rc;
+    rc = read_states(e);
+    if (rc) return rc;
+    long long att = 0;
+    for (size_t i = 0; i < e->clouds.size(); ++i) att += e->h_state[i].attempts;
+    e->attempts_at_order = att;
+    return reorder(e);
 }
 
 int sbk_get_state(sbk_engine *e, int cloud, double *pos, double *vel)
@@ -788,11 +980,13 @@ int sbk_get_state(sbk_engine *e, int cloud, double *pos, double *vel)
                                sizeof(double) * c.n, hipMemcpyDeviceToHost, e->stream));
     }
     HIP_TRY(hipStreamSynchronize(e->stream));
-    for (int i = 0; i < c.n; ++i)
+    for (int i = 0; i < c.n; ++i) {
+        const size_t o = 3 * (size_t)caller_index(c, i);
         for (int k = 0; k < 3; ++k) {
-            if (pos) pos[3 * i + k] = buf[k * (size_t)c.n + i];
-            if (vel) vel[3 * i + k] = buf[(3 + k) * (size_t)c.n + i];
+            if (pos) pos[o + k] = buf[k * (size_t)c.n + i];
+            if (vel) vel[o + k] = buf[(3 + k) * (size_t)c.n + i];
         }
+    }
     return 0;
 }
 
@@ -889,8 +1083,10 @@ static int eval_common(sbk_engine *e, int cloud, int precision, int with_fields,
                                sizeof(double) * c.n, hipMemcpyDeviceToHost, e->stream));
     rc = read_states(e);
     if (rc) return rc;
-    for (int i = 0; i < c.n; ++i)
-        for (int k = 0; k < 3; ++k) out[3 * i + k] = buf[k * (size_t)c.n + i];
+    for (int i = 0; i < c.n; ++i) {
+        const size_t o = 3 * (size_t)caller_index(c, i);
+        for (int k = 0; k < 3; ++k) out[o + k] = buf[k * (size_t)c.n + i];
+    }
     return 0;
 }
 
@@ -905,17 +1101,19 @@ int sbk_eval_coulomb(sbk_engine *e, int cloud, int precision, double *out)
     return eval_common(e, cloud, precision, 0, out);
 }
 
-int sbk_set_launch(sbk_engine *e, int tpt, int groups, int slices)
+int sbk_set_launch(sbk_engine *e, int tpt, int groups, int slices, int local)
 {
     if (!e) return fail(SBK_EINVAL, "null engine");
     if (tpt != 1 && tpt != 2 && tpt != 4 && tpt != 8)
         return fail(SBK_EINVAL, "targets_per_thread must be 1, 2, 4 or 8");
     if (groups != 1 && groups != 4) return fail(SBK_EINVAL, "source_groups must be 1 or 4");
     if (slices < 0) return fail(SBK_EINVAL, "slices must be >= 0");
-    if (tpt != e->tpt || groups != e->groups || slices != e->slices_req) {
+    if (local < -1 || local > 1) return fail(SBK_EINVAL, "local_coords must be -1, 0 or 1");
+    if (tpt != e->tpt || groups != e->groups || slices != e->slices_req || local != e->local_req) {
         e->tpt = tpt;
         e->groups = groups;
         e->slices_req = slices;
+        e->local_req = local;
         destroy_graph(e);
     }
     return 0;
@@ -931,6 +1129,7 @@ static int time_coulomb_pass(sbk_engine *e, int precision, int reps, bool with_s
     if (rc) return rc;
     sbk::CoulombArgs a = coulomb_args(e, *L, e->dev.P0, false);
     if (precision == SBK_FP64) a.eps_d = 1e-40;
+    a.local_coords = local_coords(e, precision);
     sbk::EngineDev d = e->dev;
     d.slices = L->d_slices;
     hipEvent_t t0, t1;
@@ -964,7 +1163,7 @@ int sbk_time_coulomb(sbk_engine *e, int precision, int reps, double *ms_per_laun
 }
 
 int sbk_autotune(sbk_engine *e, int precision, int *tpt_out, int *groups_out, int *slices_out,
-                 const char *table)
+                 int *local_out, const char *table)
 {
     int rc = prepare(e);
     if (rc) return rc;
@@ -972,61 +1171,72 @@ int sbk_autotune(sbk_engine *e, int precision, int *tpt_out, int *groups_out, in
         if (tpt_out) *tpt_out = 1;
         if (groups_out) *groups_out = 1;
         if (slices_out) *slices_out = 1;
+        if (local_out) *local_out = 0;
         return 0;
     }
     const int tpts[3] = {1, 2, 4};
     const int qs[2] = {1, 4};
     const int rounds[6] = {0, 1, 2, 3, 4, 8};   // 0: automatic split
-    int best_t = e->tpt, best_q = e->groups, best_s = e->slices_req;
+    const bool may_local = e->spatial && (precision == SBK_MIXED || precision == SBK_MIXED_KAHAN);
+    int best_t = e->tpt, best_q = e->groups, best_s = e->slices_req, best_l = e->local_req;
     double best = 1e300;
     char dev_name[128] = {0};
     sbk_device_info(dev_name, sizeof(dev_name), nullptr, nullptr);
     FILE *fp = table ? std::fopen(table, "a") : nullptr;
-    const int old_t = e->tpt, old_q = e->groups, old_s = e->slices_req;
-    for (int t : tpts) {
-        for (int q : qs) {
-            long long blocks = 0;
-            const int blk = sbk::kBlock / q * t;
-            for (auto &c : e->clouds)
-                if (c.n > 0) blocks += (c.n + blk - 1) / blk;
-            const long long capacity =
-                (long long)sbk::coulomb_blocks_per_cu(precision, t, e->unit_charges ? 1 : 0, q) *
-                e->cus;
-            for (int k : rounds) {
-                const int slices =
-                    k == 0 ? 0 : (int)std::max(1LL, (k * capacity + blocks / 2) / std::max(1LL, blocks));
-                e->tpt = t;
-                e->groups = q;
-                e->slices_req = slices;
-                double ms = 0.0;
-                rc = time_coulomb_pass(e, precision, 1, true, &ms);   // warm-up, list build
-                if (rc) break;
-                rc = time_coulomb_pass(e, precision, 3, true, &ms);
-                if (rc) break;
-                if (fp)
-                    std::fprintf(fp, "%s\t%d\t%zu\t%d\t%d\t%d\t%d\t%.6f\n", dev_name, e->ntot,
-                                 e->clouds.size(), precision, t, q, slices, ms);
-                if (ms < best) {
-                    best = ms;
-                    best_t = t;
-                    best_q = q;
-                    best_s = slices;
+    const int old_t = e->tpt, old_q = e->groups, old_s = e->slices_req, old_l = e->local_req;
+    for (int l = 0; l <= (may_local ? 1 : 0) && !rc; ++l) {
+        e->local_req = l;
+        for (int t : tpts) {
+            for (int q : qs) {
+                long long blocks = 0;
+                const int blk = sbk::kBlock / q * t;
+                for (auto &c : e->clouds)
+                    if (c.n > 0) blocks += (c.n + blk - 1) / blk;
+                const long long capacity =
+                    (long long)sbk::coulomb_blocks_per_cu(precision, t, e->unit_charges ? 1 : 0, q,
+                                                          local_coords(e, precision)) *
+                    e->cus;
+                for (int k : rounds) {
+                    const int slices = k == 0 ? 0
+                                              : (int)std::max(1LL, (k * capacity + blocks / 2) /
+                                                                       std::max(1LL, blocks));
+                    e->tpt = t;
+                    e->groups = q;
+                    e->slices_req = slices;
+                    double ms = 0.0;
+                    rc = time_coulomb_pass(e, precision, 1, true, &ms);   // warm-up, list build
+                    if (rc) break;
+                    rc = time_coulomb_pass(e, precision, 3, true, &ms);
+                    if (rc) break;
+                    if (fp)
+                        std::fprintf(fp, "%s\t%d\t%zu\t%d\t%d\t%d\t%d\t%d\t%.6f\n", dev_name,
+                                     e->ntot, e->clouds.size(), precision, t, q, slices,
+                                     local_coords(e, precision), ms);
+                    if (ms < best) {
+                        best = ms;
+                        best_t = t;
+                        best_q = q;
+                        best_s = slices;
+                        best_l = l;
+                    }
                 }
+                if (rc) break;
             }
             if (rc) break;
         }
-        if (rc) break;
     }
     if (fp) std::fclose(fp);
     e->tpt = old_t;
     e->groups = old_q;
     e->slices_req = old_s;
+    e->local_req = old_l;
     if (rc) return rc;
-    rc = sbk_set_launch(e, best_t, best_q, best_s);
+    rc = sbk_set_launch(e, best_t, best_q, best_s, best_l);
     if (rc) return rc;
     if (tpt_out) *tpt_out = best_t;
     if (groups_out) *groups_out = best_q;
     if (slices_out) *slices_out = best_s;
+    if (local_out) *local_out = local_coords(e, precision);
     return 0;
 }
 
@@ -1040,13 +1250,27 @@ int sbk_load_tuning(sbk_engine *e, int precision, const char *table)
     char dev_name[128] = {0};
     sbk_device_info(dev_name, sizeof(dev_name), nullptr, nullptr);
     char line[512], dev[256];
-    int ions, clouds, prec, t, q, sl;
+    int ions, clouds, prec, t, q, sl, lc;
     double ms, best = 1e300;
-    int bt = 0, bq = 0, bs = 0;
+    int bt = 0, bq = 0, bs = 0, bl = 0;
     while (std::fgets(line, sizeof(line), fp)) {
-        if (std::sscanf(line, "%255s %d %d %d %d %d %d %lf", dev, &ions, &clouds, &prec, &t, &q,
-                        &sl, &ms) != 8)
+        char *rest = nullptr;
+        int used = 0;
+        if (std::sscanf(line, "%255s %d %d %d %d %d %d%n", dev, &ions, &clouds, &prec, &t, &q,
+                        &sl, &used) != 7)
             continue;
+        rest = line + used;
+        double f1 = 0.0, f2 = 0.0;
+        const int nf = std::sscanf(rest, "%lf %lf", &f1, &f2);
+        if (nf == 2) {            // ... local_coords ms
+            lc = (int)f1;
+            ms = f2;
+        } else if (nf == 1) {     // older lines without the local_coords column
+            lc = 0;
+            ms = f1;
+        } else {
+            continue;
+        }
         if (std::strcmp(dev, dev_name) || ions != e->ntot || clouds != (int)e->clouds.size() ||
             prec != precision)
             continue;
@@ -1055,11 +1279,12 @@ int sbk_load_tuning(sbk_engine *e, int precision, const char *table)
             bt = t;
             bq = q;
             bs = sl;
+            bl = lc;
         }
     }
     std::fclose(fp);
     if (best == 1e300) return fail(SBK_ESTATE, "no matching line in %s", table);
-    return sbk_set_launch(e, bt, bq, bs);
+    return sbk_set_launch(e, bt, bq, bs, bl);
 }
 
 }  // extern "C"

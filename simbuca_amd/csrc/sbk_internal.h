@@ -80,6 +80,8 @@ struct CoulombArgs {
     int unit_weights;             // every source weight is exactly 1 (charge +1, mj = 1)
     const CloudState *state;      // nullptr: every item active
     int skip_valid_k1;            // stage 1: skip clouds whose KA[1] is still valid
+    int local_coords;             // mixed modes: far sub-tiles use FP32 offsets from the
+                                  // target box centre (pays off with ions in spatial order)
     float eps_f;
     double eps_d;
 };
@@ -110,12 +112,16 @@ enum KernelPrecision { KP_FP64_EXACT = 0, KP_FP64 = 1, KP_MIXED = 2, KP_MIXED_KA
 int launch_coulomb(int precision, int targets_per_thread, int q, const CoulombArgs &a,
                    void *stream);
 // resident work-groups per CU of the Coulomb kernel a launch would use
-int coulomb_blocks_per_cu(int precision, int targets_per_thread, int unit_weights, int q);
+int coulomb_blocks_per_cu(int precision, int targets_per_thread, int unit_weights, int q,
+                          int local_coords);
 int launch_stage(int stage, const EngineDev &d, void *stream);
 int launch_controller(const EngineDev &d, void *stream);
 int launch_commit(const EngineDev &d, void *stream);
 int launch_begin(const EngineDev &d, double duration, int budget, int set_end, void *stream);
 int launch_eval_derivs(const EngineDev &d, int with_fields, void *stream);
+// out[g] = in[src[g]] for g < n
+int launch_gather(const int *src, const double *in, double *out, int n, void *stream);
+int launch_gather_f32(const int *src, const float *in, float *out, int n, void *stream);
 
 }  // namespace sbk
 

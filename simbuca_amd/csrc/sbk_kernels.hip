@@ -11,6 +11,8 @@
 #include <float.h>
 #include <math.h>
 
+#include <type_traits>
+
 #include "sbk_internal.h"
 
 namespace sbk {
@@ -95,16 +97,47 @@ __global__ __launch_bounds__(kBlock) void coulomb_fp64_exact(CoulombArgs a)
 // and the sum over kSubTile sources are FP32; the sub-tile sums are folded
 // into FP64 (KAHAN: compensated FP32) accumulators.
 // FP64: the same tiling in FP64 with 1/sqrt.
+// LOC (mixed modes only): the work-group takes the centre o of its targets'
+// bounding box (half-diagonal h) as origin, and each source also enters LDS
+// as the FP32 offset x_s - o (one FP64 subtraction per source per
+// work-group).  A source is far when |x_s - o| >= (1 + kNearFactor) h; then
+// every separation from it is at least kNearFactor h long, and the FP32
+// difference of two offsets has an error of at most
+// 2^-24 (2 + 2/kNearFactor) |r|, against 2^-24 |r| for the FP64-formed
+// separation.  A 64-source sub-tile whose sources are all far (one ballot
+// of the wave that staged it) takes the offset path, 3 instead of 6 vector
+// instructions per pair; the others keep the FP64-formed separation.  With
+// ions in a spatial order most sub-tiles are far; in any other order nearly
+// none is, so the accuracy never depends on the order, only the speed does.
 // The self pair has r = 0 and contributes exactly 0 through the eps term.
 // ---------------------------------------------------------------------------
-template <int R, int MODE, bool UNIT, int Q>
+constexpr float kNearFactor = 0.25f;
+
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+template <int R, int MODE, bool UNIT, int Q, bool LOC>
 __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
 {
+    static_assert(!LOC || MODE != KP_FP64, "local coordinates are a mixed-precision variant");
     constexpr int L = kBlock / Q;          // lanes that share one target set
     constexpr int SPAN = kBlock / Q;       // sources of a tile handled by one wave group
     __shared__ double lx[kBlock], ly[kBlock], lz[kBlock];
-    __shared__ float lqf[kBlock];
+    __shared__ float lqf[(LOC || UNIT) ? 1 : kBlock];
     __shared__ double lqd[(MODE == KP_FP64) ? kBlock : 1];
+    __shared__ float4 lf[LOC ? kBlock : 1];                 // LOC: (x_s - o, weight)
+    __shared__ int lfar[LOC ? kBlock / kSubTile : 1];       // LOC: sub-tile is far
 
     const CoulombItem it = a.items[xcd_item(blockIdx.x, gridDim.x)];
     if (a.state && (!a.state[it.cloud].active || (a.skip_valid_k1 && a.state[it.cloud].k1_valid)))
@@ -129,10 +162,72 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
     const float epsf = a.eps_f;
     const double epsd = a.eps_d;
 
+    // LOC: origin = centre of the targets' bounding box (the same on every
+    // thread), targets as FP32 offsets from it
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    float fx[R], fy[R], fz[R];
+    float far2 = 0.f;   // LOC: squared offset beyond which a source is far
+    if (LOC) {
+        double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (it.i0 + r * L + lane < it.i1) {
+                mn[0] = fmin(mn[0], px[r]); mx[0] = fmax(mx[0], px[r]);
+                mn[1] = fmin(mn[1], py[r]); mx[1] = fmax(mx[1], py[r]);
+                mn[2] = fmin(mn[2], pz[r]); mx[2] = fmax(mx[2], pz[r]);
+            }
+        }
+        const int wave = tid >> 6;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = wave_min(mn[k]);
+            mx[k] = wave_max(mx[k]);
+        }
+        if ((tid & 63) == 0) {
+            lx[wave] = mn[0]; ly[wave] = mn[1]; lz[wave] = mn[2];
+            lx[4 + wave] = mx[0]; ly[4 + wave] = mx[1]; lz[4 + wave] = mx[2];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) {
+            mn[0] = fmin(mn[0], lx[w]); mx[0] = fmax(mx[0], lx[4 + w]);
+            mn[1] = fmin(mn[1], ly[w]); mx[1] = fmax(mx[1], ly[4 + w]);
+            mn[2] = fmin(mn[2], lz[w]); mx[2] = fmax(mx[2], lz[4 + w]);
+        }
+        ox = 0.5 * (mn[0] + mx[0]);
+        oy = 0.5 * (mn[1] + mx[1]);
+        oz = 0.5 * (mn[2] + mx[2]);
+        const float hx = (float)(0.5 * (mx[0] - mn[0]));
+        const float hy = (float)(0.5 * (mx[1] - mn[1]));
+        const float hz = (float)(0.5 * (mx[2] - mn[2]));
+        far2 = (1.f + kNearFactor) * (1.f + kNearFactor) * (hx * hx + hy * hy + hz * hz);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            fx[r] = (float)(px[r] - ox);
+            fy[r] = (float)(py[r] - oy);
+            fz[r] = (float)(pz[r] - oz);
+        }
+    }
+
     for (int t0 = it.j0; t0 < it.j1; t0 += kBlock) {
         const int cnt = min(kBlock, it.j1 - t0);
         __syncthreads();
-        if (tid < cnt) {
+        if (LOC) {
+            // offsets; wave w stages sub-tile w and decides whether it is far
+            bool far_src = true;
+            if (tid < cnt) {
+                const double sx = a.sx[t0 + tid], sy = a.sy[t0 + tid], sz = a.sz[t0 + tid];
+                lx[tid] = sx;
+                ly[tid] = sy;
+                lz[tid] = sz;
+                const float4 s4 = make_float4((float)(sx - ox), (float)(sy - oy), (float)(sz - oz),
+                                              UNIT ? 1.f : a.sqf[t0 + tid]);
+                lf[tid] = s4;
+                far_src = fmaf(s4.x, s4.x, fmaf(s4.y, s4.y, s4.z * s4.z)) >= far2;
+            }
+            const unsigned long long all_far = __ballot(far_src);
+            if ((tid & 63) == 0) lfar[tid >> 6] = all_far == ~0ull ? 1 : 0;
+        } else if (tid < cnt) {
             lx[tid] = a.sx[t0 + tid];
             ly[tid] = a.sy[t0 + tid];
             lz[tid] = a.sz[t0 + tid];
@@ -165,14 +260,22 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
 #pragma unroll
                 for (int r = 0; r < R; ++r) ax[r] = ay[r] = az[r] = 0.f;
                 const int m = min(kSubTile, b1 - jj);
-                auto body = [&](int j) {
-                    const double sx = lx[j], sy = ly[j], sz = lz[j];
-                    const float q = lqf[j];
+                auto body = [&](int j, auto far_tag) {
+                    constexpr bool FAR = decltype(far_tag)::value;
+                    double sx = 0.0, sy = 0.0, sz = 0.0;
+                    float4 s = make_float4(0.f, 0.f, 0.f, 1.f);
+                    if (FAR) {
+                        s = lf[j];
+                    } else {
+                        sx = lx[j]; sy = ly[j]; sz = lz[j];
+                        if (!UNIT) s.w = LOC ? lf[j].w : lqf[j];
+                    }
+                    const float q = s.w;
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
-                        const float dx = (float)(sx - px[r]);
-                        const float dy = (float)(sy - py[r]);
-                        const float dz = (float)(sz - pz[r]);
+                        const float dx = FAR ? s.x - fx[r] : (float)(sx - px[r]);
+                        const float dy = FAR ? s.y - fy[r] : (float)(sy - py[r]);
+                        const float dz = FAR ? s.z - fz[r] : (float)(sz - pz[r]);
                         float d2 = fmaf(dz, dz, epsf);
                         d2 = fmaf(dy, dy, d2);
                         d2 = fmaf(dx, dx, d2);
@@ -184,11 +287,31 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
                         az[r] = fmaf(dz, w, az[r]);
                     }
                 };
-                if (m == kSubTile) {
+                auto run = [&](auto far_tag) {
+                    constexpr bool FAR = decltype(far_tag)::value;
+                    if (m == kSubTile) {
+                        if constexpr (FAR || !LOC) {
 #pragma unroll 8
-                    for (int j = 0; j < kSubTile; ++j) body(jj + j);
+                            for (int j = 0; j < kSubTile; ++j) body(jj + j, far_tag);
+                        } else {
+                            // the minority path of LOC: a shorter unroll keeps the
+                            // kernel's register count at the far path's
+#pragma unroll 2
+                            for (int j = 0; j < kSubTile; ++j) body(jj + j, far_tag);
+                        }
+                    } else {
+                        for (int j = 0; j < m; ++j) body(jj + j, far_tag);
+                    }
+                };
+                int far = 0;
+                if (LOC) far = __builtin_amdgcn_readfirstlane(lfar[jj / kSubTile]);
+                if constexpr (LOC) {
+                    if (far)
+                        run(std::true_type{});
+                    else
+                        run(std::false_type{});
                 } else {
-                    for (int j = 0; j < m; ++j) body(jj + j);
+                    run(std::false_type{});
                 }
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
@@ -564,6 +687,14 @@ __global__ __launch_bounds__(256) void eval_derivs_kernel(EngineDev d, int with_
     d.derivs_out[2 * d.stride + g] = Fz;
 }
 
+// out[g] = in[src[g]]: applies a new ion order to one per-ion array.
+template <typename T>
+__global__ __launch_bounds__(256) void gather_kernel(const int *src, const T *in, T *out, int n)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) out[g] = in[src[g]];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -573,40 +704,47 @@ static inline int check_launch()
 }
 
 template <int R, bool UNIT, int Q>
-static const void *tiled_fn_q(int precision)
+static const void *tiled_fn_q(int precision, int loc)
 {
     switch (precision) {
-    case KP_FP64: return reinterpret_cast<const void *>(&coulomb_tiled<R, KP_FP64, false, Q>);
-    case KP_MIXED: return reinterpret_cast<const void *>(&coulomb_tiled<R, KP_MIXED, UNIT, Q>);
+    case KP_FP64:
+        return loc ? nullptr : reinterpret_cast<const void *>(&coulomb_tiled<R, KP_FP64, false, Q, false>);
+    case KP_MIXED:
+        return loc ? reinterpret_cast<const void *>(&coulomb_tiled<R, KP_MIXED, UNIT, Q, true>)
+                   : reinterpret_cast<const void *>(&coulomb_tiled<R, KP_MIXED, UNIT, Q, false>);
     case KP_MIXED_KAHAN:
-        return reinterpret_cast<const void *>(&coulomb_tiled<R, KP_MIXED_KAHAN, UNIT, Q>);
+        return loc ? reinterpret_cast<const void *>(&coulomb_tiled<R, KP_MIXED_KAHAN, UNIT, Q, true>)
+                   : reinterpret_cast<const void *>(&coulomb_tiled<R, KP_MIXED_KAHAN, UNIT, Q, false>);
     default: return nullptr;
     }
 }
 
 template <int R>
-static const void *tiled_fn(int precision, int unit, int q)
+static const void *tiled_fn(int precision, int unit, int q, int loc)
 {
-    if (q == 4) return unit ? tiled_fn_q<R, true, 4>(precision) : tiled_fn_q<R, false, 4>(precision);
-    if (q == 1) return unit ? tiled_fn_q<R, true, 1>(precision) : tiled_fn_q<R, false, 1>(precision);
+    if (q == 4)
+        return unit ? tiled_fn_q<R, true, 4>(precision, loc) : tiled_fn_q<R, false, 4>(precision, loc);
+    if (q == 1)
+        return unit ? tiled_fn_q<R, true, 1>(precision, loc) : tiled_fn_q<R, false, 1>(precision, loc);
     return nullptr;
 }
 
-static const void *coulomb_fn(int precision, int tpt, int unit, int q)
+static const void *coulomb_fn(int precision, int tpt, int unit, int q, int loc)
 {
-    if (precision == KP_FP64_EXACT) return reinterpret_cast<const void *>(&coulomb_fp64_exact);
+    if (precision == KP_FP64_EXACT)
+        return loc ? nullptr : reinterpret_cast<const void *>(&coulomb_fp64_exact);
     switch (tpt) {
-    case 1: return tiled_fn<1>(precision, unit, q);
-    case 2: return tiled_fn<2>(precision, unit, q);
-    case 4: return tiled_fn<4>(precision, unit, q);
-    case 8: return tiled_fn<8>(precision, unit, q);
+    case 1: return tiled_fn<1>(precision, unit, q, loc);
+    case 2: return tiled_fn<2>(precision, unit, q, loc);
+    case 4: return tiled_fn<4>(precision, unit, q, loc);
+    case 8: return tiled_fn<8>(precision, unit, q, loc);
     default: return nullptr;
     }
 }
 
-int coulomb_blocks_per_cu(int precision, int tpt, int unit, int q)
+int coulomb_blocks_per_cu(int precision, int tpt, int unit, int q, int loc)
 {
-    const void *f = coulomb_fn(precision, tpt, unit, q);
+    const void *f = coulomb_fn(precision, tpt, unit, q, loc);
     if (!f) return 1;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kBlock, 0) != hipSuccess || nb < 1)
@@ -617,7 +755,7 @@ int coulomb_blocks_per_cu(int precision, int tpt, int unit, int q)
 int launch_coulomb(int precision, int tpt, int q, const CoulombArgs &a, void *stream)
 {
     if (a.nitems <= 0) return 0;
-    const void *f = coulomb_fn(precision, tpt, a.unit_weights, q);
+    const void *f = coulomb_fn(precision, tpt, a.unit_weights, q, a.local_coords);
     if (!f) return -1;
     void *args[] = {const_cast<CoulombArgs *>(&a)};
     if (hipLaunchKernel(f, dim3(a.nitems), dim3(kBlock), args, 0, (hipStream_t)stream) !=
@@ -662,6 +800,22 @@ int launch_begin(const EngineDev &d, double duration, int budget, int set_end, v
 {
     hipLaunchKernelGGL(begin_kernel, dim3((d.nclouds + 255) / 256), dim3(256), 0,
                        (hipStream_t)stream, d, duration, budget, set_end);
+    return check_launch();
+}
+
+int launch_gather(const int *src, const double *in, double *out, int n, void *stream)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(gather_kernel<double>, dim3((n + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, src, in, out, n);
+    return check_launch();
+}
+
+int launch_gather_f32(const int *src, const float *in, float *out, int n, void *stream)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(gather_kernel<float>, dim3((n + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, src, in, out, n);
     return check_launch();
 }
 

@@ -80,7 +80,7 @@ void build_items(int n)
     const int blk = sbk::kBlock / kGroups * kTpt;
     const int ib = (n + blk - 1) / blk;
     const long long capacity =
-        (long long)sbk::coulomb_blocks_per_cu(sbk::KP_MIXED, kTpt, 1, kGroups) * g.cus;
+        (long long)sbk::coulomb_blocks_per_cu(sbk::KP_MIXED, kTpt, 1, kGroups, 0) * g.cus;
     int S = (int)std::max(1LL, std::min<long long>((8 * capacity + ib / 2) / ib, std::max(1, n / 512)));
     int chunk = (n + S - 1) / S;
     chunk = ((chunk + sbk::kSubTile - 1) / sbk::kSubTile) * sbk::kSubTile;

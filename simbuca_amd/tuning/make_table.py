@@ -4,7 +4,8 @@ The reference ships per-GPU launch tables read at start-up
 (libraries/NBODY/<GPU>_NB0DY.dat, read by bin/nbody.cpp:275-321).  This is
 their counterpart: sbk_autotune times every launch shape on the benchmark
 workloads, and the lines ("device ions clouds precision targets/thread
-source-groups slices ms") are written to simbuca_amd/tuning/<arch>.tsv.
+source-groups slices local-coords ms") are written to
+simbuca_amd/tuning/<arch>.tsv.
 The engine applies the fastest line that matches its device, ion count,
 cloud count and precision with sbk_load_tuning.
 

@@ -93,32 +93,58 @@ def test_force_time_dependent_terms(rng):
     assert np.allclose(out, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6), ("mixed_kahan", 1e-6)])
-def test_coulomb_precisions_at_65536(precision, tol):
-    w = synth.workload(3)
-    c = w.clouds[0]
+@pytest.fixture(scope="module")
+def exact_65536():
+    """FP64-exact Coulomb sum of the 65,536-ion cloud, summed in the caller's
+    order (FP64-exact engines never reorder), checked against oracle rows."""
+    c = synth.workload(3).clouds[0]
     rows = np.random.Generator(np.random.PCG64(7)).choice(c.n, 192, replace=False).astype(np.int32)
     ref = O.coulomb_rows(c.pos, c.mass, c.charge, 1.0, rows)
-    with S.Engine(precision=precision) as e:
+    with S.Engine(precision="fp64_exact") as e:
         e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
-        f = e.eval_coulomb(0, precision)
         exact = e.eval_coulomb(0, "fp64_exact")
     assert np.array_equal(exact[rows], ref)
-    err = _rows_error(f, exact)
+    return exact
+
+
+@pytest.mark.parametrize("precision,local,tol", [("fp64", 0, 1e-13), ("mixed", 0, 1e-6),
+                                                 ("mixed", 1, 1e-6), ("mixed_kahan", 0, 1e-6),
+                                                 ("mixed_kahan", 1, 1e-6)])
+def test_coulomb_precisions_at_65536(exact_65536, precision, local, tol):
+    c = synth.workload(3).clouds[0]
+    with S.Engine(precision=precision) as e:
+        e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
+        e.set_launch(4, 4, 0, local)
+        f = e.eval_coulomb(0, precision)
+    err = _rows_error(f, exact_65536)
     assert err <= tol, err
     # Newton's third law holds to the same accuracy (equal masses and charges)
     assert np.linalg.norm(f.sum(0)) <= 10 * tol * np.abs(f).sum(0).max()
 
 
+def test_local_coords_keep_the_accuracy(exact_65536):
+    # far sub-tiles from FP32 offsets: same normwise error as FP64-formed
+    # separations everywhere (measured 5.6e-8 both), well inside 2x
+    c = synth.workload(3).clouds[0]
+    errs = {}
+    with S.Engine(precision="mixed") as e:
+        e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
+        for local in (0, 1):
+            e.set_launch(4, 4, 0, local)
+            errs[local] = _rows_error(e.eval_coulomb(0, "mixed"), exact_65536)
+    assert errs[1] <= 2.0 * errs[0], errs
+
+
 @pytest.mark.parametrize("tpt", [1, 2, 4])
 @pytest.mark.parametrize("groups", [1, 4])
 @pytest.mark.parametrize("slices", [0, 1, 3, 7])
-def test_launch_shapes_agree(rng, tpt, groups, slices):
+@pytest.mark.parametrize("local", [0, 1])
+def test_launch_shapes_agree(rng, tpt, groups, slices, local):
     amu, m, ch, pos, vel = _rand_cloud(rng, 3001, sigma=1e-3)
     sf, _ = _both_fields(trap=0, coulomb=1)
     with S.Engine(precision="mixed") as e:
         e.add_cloud(pos, vel, m, ch, sf)
-        e.set_launch(tpt, groups, slices)
+        e.set_launch(tpt, groups, slices, local)
         mixed = e.eval_coulomb(0, "mixed")
         fp64 = e.eval_coulomb(0, "fp64")
         exact = e.eval_coulomb(0, "fp64_exact")
@@ -135,13 +161,73 @@ def test_autotune_table_round_trip(rng, tmp_path):
         shape = e.autotune("mixed", table)
         ref = e.eval_coulomb(0, "mixed")
     lines = open(table).read().strip().splitlines()
-    assert len(lines) == 3 * 2 * 6
+    assert len(lines) == 2 * 3 * 2 * 6                 # local coords x R x Q x slice rounds
+    assert {line.split()[7] for line in lines} == {"0", "1"}
     with S.Engine(precision="mixed") as e:
         e.add_cloud(pos, vel, m, ch, sf)
         assert e.load_tuning(table, "mixed")
         assert not e.load_tuning(table, "fp64")        # no FP64 lines in the table
         again = e.eval_coulomb(0, "mixed")
     assert np.array_equal(again, ref)                  # same shape -> same sums
+
+
+def test_spatial_order_is_invisible_to_the_caller(rng):
+    # mixed engines keep each cloud in a spatial order on the device; states
+    # and forces come back in the caller's order, and re-sorting an unmoved
+    # cloud changes nothing at all
+    amu, m, ch, pos, vel = _rand_cloud(rng, 5000, sigma=1e-3)
+    kw = dict(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0)
+    sf, of = _both_fields(**kw)
+    assert not np.array_equal(S.spatial_order(pos), np.arange(5000))
+    with S.Engine(precision="mixed") as e:
+        e.add_cloud(pos, vel, m, ch, sf)
+        e.set_launch(4, 4, 0, 1)
+        gp, gv = e.state(0)
+        f = e.eval_force(0)
+        c = e.eval_coulomb(0, "mixed")
+        e.reorder()
+        gp2, gv2 = e.state(0)
+        f2 = e.eval_force(0)
+    assert np.array_equal(gp, pos) and np.array_equal(gv, vel)
+    assert np.array_equal(gp2, pos) and np.array_equal(gv2, vel)
+    assert np.array_equal(f, f2)
+    ref = O.coulomb_rows(pos, m, ch, 1.0, np.arange(5000, dtype=np.int32))
+    assert _rows_error(c, ref) <= 1e-6
+    assert _rows_error(f, O.force(pos, vel, m, ch, of, 0.0)) <= 1e-6
+
+
+@pytest.mark.parametrize("precision", ["mixed", "mixed_kahan"])
+def test_reorder_during_a_run(rng, precision):
+    # re-sorting after every batch of attempts (ions move) against never
+    # re-sorting, over 40 fixed 1 ns steps (fixed steps: an adaptive run ends
+    # where its last step lands, which rounding moves): same Coulomb
+    # evaluations (the stage-1 derivative moves with its ion), and states
+    # equal up to the mixed precision's summation order, which moves a
+    # position by ~|a| * 5e-8 * t^2 / 2 ~ 1e-15 m here
+    amu, m, ch, pos, vel = _rand_cloud(rng, 3000, sigma=3e-4, species=(133.0,), charges=(1,))
+    kw = dict(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0, Ed=20.0, fd=3e5)
+    sf, of = _both_fields(**kw)
+    res = []
+    for interval in (0, 1):
+        with S.Engine(precision=precision, adaptive=False, h0=1e-9, time_mode=S.TIME_STAGE,
+                      reorder_interval=interval) as e:
+            e.add_cloud(pos, vel, m, ch, sf)
+            e.set_launch(1, 4, 0, 1)
+            e.advance(3.95e-8)
+            res.append((*e.state(0), e.status(0)))
+    (p0, v0, s0), (p1, v1, s1) = res
+    assert s0["attempts"] == s1["attempts"] == 40
+    assert s0["coulomb_evals"] == s1["coulomb_evals"] == 7 + 6 * 39
+    assert s0["lifetime"] == s1["lifetime"]
+    assert np.allclose(p1, p0, rtol=0, atol=1e-13)
+    assert np.allclose(v1, v0, rtol=0, atol=1e-5)
+    # and both follow the FP64 oracle to the mixed precision's accuracy
+    c = O.Cloud(pos, vel, m, ch, of, h0=1e-9, adaptive=False, time_mode=O.TIME_STAGE)
+    for _ in range(40):
+        c.step()
+    op, ov, _ = c.state()
+    assert np.allclose(p1, op, rtol=0, atol=1e-13)
+    assert np.allclose(v1, ov, rtol=0, atol=1e-5)
 
 
 def test_multi_cloud_equals_single_clouds(rng):
