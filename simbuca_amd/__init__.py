@@ -275,7 +275,7 @@ class Engine:
     def set_step(self, cloud, h):
         _check(lib().sbk_set_step(self._h, cloud, float(h)))
 
-    def set_launch(self, targets_per_thread=4, source_groups=4, slices=0, local_coords=-1):
+    def set_launch(self, targets_per_thread=0, source_groups=4, slices=0, local_coords=-1):
         _check(lib().sbk_set_launch(self._h, int(targets_per_thread), int(source_groups),
                                     int(slices), int(local_coords)))
 

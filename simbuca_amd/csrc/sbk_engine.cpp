@@ -101,8 +101,8 @@ struct sbk_engine {
     long long reorders = 0;
     sbk::EngineDev dev{};
 
-    // launch shape
-    int tpt = 4;
+    // launch shape (tpt 0: automatic)
+    int tpt = 0;
     int groups = 4;
     int slices_req = 0;
     int local_req = -1;          // -1: automatic
@@ -259,12 +259,28 @@ int local_coords(const sbk_engine *e, int precision)
     return n2 > 0.0 && n3 / n2 >= 32768.0 ? 1 : 0;
 }
 
+// Targets per thread.  0 (the default) picks by size, fitted to the MI355X
+// tuning table: 4 once a cloud fills the device with long work items (pair-
+// weighted mean size >= 32768), 2 for ensembles with many pairs in total,
+// 1 for small clouds, where more, shorter work-groups win.
+int targets_per_thread(const sbk_engine *e)
+{
+    if (e->tpt > 0) return e->tpt;
+    double n2 = 0.0, n3 = 0.0;
+    for (auto &c : e->clouds) {
+        n2 += (double)c.n * c.n;
+        n3 += (double)c.n * c.n * c.n;
+    }
+    if (n2 > 0.0 && n3 / n2 >= 32768.0) return 4;
+    return n2 >= 2.5e8 ? 2 : 1;
+}
+
 // Builds (or returns the cached) Coulomb work list for a precision and
 // launch shape.  `only_coulomb` keeps the clouds whose Coulomb switch is on.
 int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
 {
     const bool exact = precision == SBK_FP64_EXACT;
-    const int tpt = exact ? 1 : e->tpt;
+    const int tpt = exact ? 1 : targets_per_thread(e);
     const int q = exact ? 1 : e->groups;
     const int sreq = exact ? 1 : e->slices_req;
     auto key = std::make_tuple(precision, tpt, q, sreq, only_coulomb ? 1 : 0,
@@ -555,7 +571,7 @@ int launch_coulomb_timed(sbk_engine *e, int precision, const sbk::CoulombArgs &a
     sbk::CoulombArgs b = a;
     if (precision == SBK_FP64) b.eps_d = 1e-40;   // the guard of bin/force.cpp:136
     b.local_coords = local_coords(e, precision);
-    if (sbk::launch_coulomb(kp(precision), e->tpt, e->groups, b, e->stream))
+    if (sbk::launch_coulomb(kp(precision), targets_per_thread(e), e->groups, b, e->stream))
         return fail(SBK_EHIP, "Coulomb kernel launch failed: %s",
                     hipGetErrorString(hipGetLastError()));
     if (evp) HIP_TRY(hipEventRecord(evp->second, e->stream));
@@ -1104,8 +1120,8 @@ int sbk_eval_coulomb(sbk_engine *e, int cloud, int precision, double *out)
 int sbk_set_launch(sbk_engine *e, int tpt, int groups, int slices, int local)
 {
     if (!e) return fail(SBK_EINVAL, "null engine");
-    if (tpt != 1 && tpt != 2 && tpt != 4 && tpt != 8)
-        return fail(SBK_EINVAL, "targets_per_thread must be 1, 2, 4 or 8");
+    if (tpt != 0 && tpt != 1 && tpt != 2 && tpt != 4 && tpt != 8)
+        return fail(SBK_EINVAL, "targets_per_thread must be 0 (automatic), 1, 2, 4 or 8");
     if (groups != 1 && groups != 4) return fail(SBK_EINVAL, "source_groups must be 1 or 4");
     if (slices < 0) return fail(SBK_EINVAL, "slices must be >= 0");
     if (local < -1 || local > 1) return fail(SBK_EINVAL, "local_coords must be -1, 0 or 1");
@@ -1137,7 +1153,7 @@ static int time_coulomb_pass(sbk_engine *e, int precision, int reps, bool with_s
     HIP_TRY(hipEventCreate(&t1));
     HIP_TRY(hipEventRecord(t0, e->stream));
     for (int i = 0; i < reps; ++i) {
-        if (sbk::launch_coulomb(kp(precision), e->tpt, e->groups, a, e->stream))
+        if (sbk::launch_coulomb(kp(precision), targets_per_thread(e), e->groups, a, e->stream))
             return fail(SBK_EHIP, "Coulomb kernel launch failed");
         if (with_sum && sbk::launch_eval_derivs(d, 0, e->stream))
             return fail(SBK_EHIP, "partial-sum kernel launch failed");
