@@ -164,6 +164,7 @@ typedef struct sbk_stats {
     long long attempts;          /* sum over clouds */
     long long steps;             /* accepted steps, summed over clouds */
     double    ion_steps;         /* sum over clouds of steps*n */
+    long long reorders;          /* re-sorts of the device ion order (spatial_order) */
 } sbk_stats;
 
 typedef struct sbk_engine sbk_engine;

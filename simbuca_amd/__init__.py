@@ -94,6 +94,7 @@ class Stats(ctypes.Structure):
         ("coulomb_ms", ctypes.c_double), ("coulomb_launches", ctypes.c_longlong),
         ("pair_interactions", ctypes.c_double), ("attempts", ctypes.c_longlong),
         ("steps", ctypes.c_longlong), ("ion_steps", ctypes.c_double),
+        ("reorders", ctypes.c_longlong),
     ]
 
     def as_dict(self):

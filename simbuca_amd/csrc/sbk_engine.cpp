@@ -98,7 +98,7 @@ struct sbk_engine {
     bool spatial = false;        // ions kept in a spatial order (mixed precisions)
     int *d_src = nullptr;        // gather indices of a re-sort
     long long attempts_at_order = 0;
-    long long reorders = 0;
+    long long reorders = 0, reorders_at_reset = 0;
     sbk::EngineDev dev{};
 
     // launch shape (tpt 0: automatic)
@@ -1043,6 +1043,7 @@ int sbk_get_stats(sbk_engine *e, sbk_stats *out)
     std::memset(out, 0, sizeof(*out));
     out->coulomb_ms = e->coulomb_ms;
     out->coulomb_launches = e->coulomb_launches;
+    out->reorders = e->reorders - e->reorders_at_reset;
     for (size_t i = 0; i < e->clouds.size(); ++i) {
         const double n = e->clouds[i].n;
         const long long att = e->h_state[i].attempts - e->attempts_at_reset[i];
@@ -1069,6 +1070,7 @@ int sbk_reset_stats(sbk_engine *e)
     }
     e->coulomb_ms = 0.0;
     e->coulomb_launches = 0;
+    e->reorders_at_reset = e->reorders;
     return 0;
 }
 

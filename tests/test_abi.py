@@ -34,6 +34,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_sizes_match_header():
     # sbk_config: 5 ints, padding to 8, 3 doubles (x86-64 ABI)
     assert ctypes.sizeof(S.Config) == 5 * 4 + 4 + 3 * 8 + 2 * 4
+    assert ctypes.sizeof(S.Stats) == 7 * 8
     assert ctypes.sizeof(S.Fields) == ctypes.sizeof(O.Fields)
     for (a, ta), (b, tb) in zip(S.Fields._fields_, O.Fields._fields_):
         assert a == b and ta == tb
