@@ -128,6 +128,21 @@ typedef struct sbk_fields {
     double delta;               /* buffer-gas friction a -= delta v [1/s] */
     int    ramp;                /* linear Vrf/Vdc ramp of bin/ion_fly.cpp:235-239 */
     double ramp_t0, ramp_t1, newVrf, newVdc;
+    /* Penning excitation menu and field extras (none exists in the reference
+     * fork; each is defined from its potential and is off when its amplitude
+     * is 0).  Phases are psi = 2 pi (f t + rate t^2/2) + phi. */
+    double Uo, ro, fo, phio;    /* octupole: potential -(Uo/(4 ro^4)) cos(psi) Re (x+iy)^4,
+                                   E = (Uo/ro^4) cos(psi) (x^3-3xy^2, y^3-3x^2y, 0) */
+    int    rw_order;            /* rotating wall of order l = 1 (dipole) or 2 (quadrupole): */
+    double Uw, rw, fw, phiw;    /* potential -(Uw/(l rw^l)) Re[(x+iy)^l e^{-i l psi}], turning at fw;
+                                   E = (Uw/rw) (cos psi, sin psi, 0) for l = 1,
+                                   E = (Uw/rw^2) (x cos 2psi + y sin 2psi, x sin 2psi - y cos 2psi, 0) for l = 2 */
+    double Eax, fax, phiax;     /* axial dipole excitation along z: Eax cos(psi) [V/m] */
+    double fd_rate, fq_rate;    /* linear sweeps of the dipole and quadrupole drive frequencies [Hz/s] */
+    double B2;                  /* magnetic bottle: B += B2 ((z^2-(x^2+y^2)/2) z_hat - z (x x_hat + y y_hat)) [T/m^2] */
+    double K0, p_mbar, T_gas;   /* buffer gas by reduced ion mobility K0 [m^2/(V s)] at pressure p_mbar [mbar]
+                                   and temperature T_gas [K]: a -= (q e/m) v / K, K = K0 (1013.25/p_mbar)(T_gas/273.15);
+                                   off when K0 or p_mbar is 0 */
 } sbk_fields;
 
 typedef struct sbk_config {

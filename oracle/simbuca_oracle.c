@@ -79,8 +79,27 @@ void orc_force(int n, const double *pos_, const double *vel_, const double *mass
     double z02 = f->z0 * f->z0;
     double omega_rf = f->freq_rf * 2 * pi;           /* force.cpp:44 */
     double cpen = (f->V0 != 0.0) ? f->V0 / (2.0 * f->d0 * f->d0) : 0.0;
-    double cdip = (f->Ed != 0.0) ? f->Ed * cos(2.0 * pi * f->fd * t + f->phid) : 0.0;
-    double cqex = (f->Uq != 0.0) ? (f->Uq / (f->rq * f->rq)) * cos(2.0 * pi * f->fq * t + f->phiq) : 0.0;
+    /* drive phases 2 pi (f t + rate t^2 / 2) + phi; rate = 0 adds an exact 0 */
+    double cdip = (f->Ed != 0.0)
+                      ? f->Ed * cos(2.0 * pi * f->fd * t + pi * f->fd_rate * t * t + f->phid) : 0.0;
+    double cqex = (f->Uq != 0.0) ? (f->Uq / (f->rq * f->rq)) *
+                                       cos(2.0 * pi * f->fq * t + pi * f->fq_rate * t * t + f->phiq)
+                                 : 0.0;
+    /* octupole, rotating wall, axial drive, buffer gas (extensions, see the header) */
+    double coct = (f->Uo != 0.0)
+                      ? (f->Uo / (f->ro * f->ro * f->ro * f->ro)) * cos(2.0 * pi * f->fo * t + f->phio)
+                      : 0.0;
+    double cw = 0.0, sw = 0.0;
+    if (f->Uw != 0.0) {
+        double psi = 2.0 * pi * f->fw * t + f->phiw;
+        double amp = (f->rw_order == 1) ? f->Uw / f->rw : f->Uw / (f->rw * f->rw);
+        double ang = (f->rw_order == 1) ? psi : 2.0 * psi;
+        cw = amp * cos(ang);
+        sw = amp * sin(ang);
+    }
+    double cax = (f->Eax != 0.0) ? f->Eax * cos(2.0 * pi * f->fax * t + f->phiax) : 0.0;
+    double invK = (f->K0 != 0.0 && f->p_mbar != 0.0)
+                      ? (f->p_mbar / 1013.25) * (273.15 / f->T_gas) / f->K0 : 0.0;
 
     for (int j_ = 0; j_ < n; j_++) {
         double x = pos[j_][0], y = pos[j_][1], z = pos[j_][2];
@@ -116,6 +135,35 @@ void orc_force(int n, const double *pos_, const double *vel_, const double *mass
             derivs[j_][0] -= f->delta * vx;
             derivs[j_][1] -= f->delta * vy;
             derivs[j_][2] -= f->delta * vz;
+        }
+        if (f->B2 != 0.0) {
+            double b2z = f->B2 * z;
+            double bx = -b2z * x, by = -b2z * y;
+            double bz = f->B2 * (z * z - 0.5 * (x * x + y * y));
+            derivs[j_][0] += qDmassa * (vy * bz - vz * by);
+            derivs[j_][1] += qDmassa * (vz * bx - vx * bz);
+            derivs[j_][2] += qDmassa * (vx * by - vy * bx);
+        }
+        if (coct != 0.0) {
+            derivs[j_][0] += qDmassa * coct * (x * (x * x - 3.0 * y * y));
+            derivs[j_][1] += qDmassa * coct * (y * (y * y - 3.0 * x * x));
+        }
+        if (f->Uw != 0.0) {
+            if (f->rw_order == 1) {
+                derivs[j_][0] += qDmassa * cw;
+                derivs[j_][1] += qDmassa * sw;
+            } else {
+                derivs[j_][0] += qDmassa * (cw * x + sw * y);
+                derivs[j_][1] += qDmassa * (sw * x - cw * y);
+            }
+        }
+        if (cax != 0.0)
+            derivs[j_][2] += qDmassa * cax;
+        if (invK != 0.0) {
+            double g = qDmassa * invK;
+            derivs[j_][0] -= g * vx;
+            derivs[j_][1] -= g * vy;
+            derivs[j_][2] -= g * vz;
         }
     }
 

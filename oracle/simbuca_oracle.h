@@ -49,6 +49,20 @@ typedef struct orc_fields {
     /* trap ramp of move_particles (ion_fly.cpp:235-239) */
     int    ramp;
     double ramp_t0, ramp_t1, newVrf, newVdc;
+    /* octupole excitation (Uo/ro^4) cos(psi) (x^3 - 3xy^2, y^3 - 3x^2y, 0) [V/m] */
+    double Uo, ro, fo, phio;
+    /* rotating wall of order rw_order (1: (Uw/rw)(cos psi, sin psi, 0);
+     * 2: (Uw/rw^2)(x cos 2psi + y sin 2psi, x sin 2psi - y cos 2psi, 0)) */
+    int    rw_order;
+    double Uw, rw, fw, phiw;
+    /* axial dipole excitation Eax cos(psi) along z */
+    double Eax, fax, phiax;
+    /* frequency sweep rates of the dipole and quadrupole drives [Hz/s] */
+    double fd_rate, fq_rate;
+    /* magnetic bottle B2 [T/m^2] */
+    double B2;
+    /* buffer gas by reduced mobility K0 at p_mbar and T_gas */
+    double K0, p_mbar, T_gas;
 } orc_fields;
 
 enum { ORC_TIME_FROZEN = 0, ORC_TIME_STAGE = 1 };

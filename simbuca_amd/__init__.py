@@ -59,6 +59,14 @@ class Fields(ctypes.Structure):
         ("delta", ctypes.c_double),
         ("ramp", ctypes.c_int), ("ramp_t0", ctypes.c_double), ("ramp_t1", ctypes.c_double),
         ("newVrf", ctypes.c_double), ("newVdc", ctypes.c_double),
+        ("Uo", ctypes.c_double), ("ro", ctypes.c_double), ("fo", ctypes.c_double),
+        ("phio", ctypes.c_double),
+        ("rw_order", ctypes.c_int), ("Uw", ctypes.c_double), ("rw", ctypes.c_double),
+        ("fw", ctypes.c_double), ("phiw", ctypes.c_double),
+        ("Eax", ctypes.c_double), ("fax", ctypes.c_double), ("phiax", ctypes.c_double),
+        ("fd_rate", ctypes.c_double), ("fq_rate", ctypes.c_double),
+        ("B2", ctypes.c_double),
+        ("K0", ctypes.c_double), ("p_mbar", ctypes.c_double), ("T_gas", ctypes.c_double),
     ]
 
     def as_dict(self):

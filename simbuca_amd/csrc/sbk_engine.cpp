@@ -149,6 +149,21 @@ sbk::CloudParams make_params(const sbk_fields &f, int off, int n)
     p.fq = f.fq;
     p.phiq = f.phiq;
     p.delta = f.delta;
+    p.fd_rate = f.fd_rate;
+    p.fq_rate = f.fq_rate;
+    p.Uo_ro4 = (f.Uo != 0.0) ? f.Uo / (f.ro * f.ro * f.ro * f.ro) : 0.0;
+    p.fo = f.fo;
+    p.phio = f.phio;
+    p.rw_order = f.rw_order;
+    p.Uw_rwl = (f.Uw != 0.0) ? ((f.rw_order == 1) ? f.Uw / f.rw : f.Uw / (f.rw * f.rw)) : 0.0;
+    p.fw = f.fw;
+    p.phiw = f.phiw;
+    p.Eax = f.Eax;
+    p.fax = f.fax;
+    p.phiax = f.phiax;
+    p.B2 = f.B2;
+    p.inv_K = (f.K0 != 0.0 && f.p_mbar != 0.0) ? (f.p_mbar / 1013.25) * (273.15 / f.T_gas) / f.K0
+                                                : 0.0;
     p.off = off;
     p.n = n;
     p.slices = 1;
@@ -374,7 +389,8 @@ sbk::CloudParams cloud_params(const sbk_engine *e, size_t ci)
     // to y_{n+1}, and with stage times t + c7 h = t + h = t_{n+1}.  With the
     // reference's frozen time it is exact only if no term depends on time; the
     // trap ramp is evaluated at the step's start time, so it never is.
-    const bool timed = (c.f.trap && c.f.Vrf != 0.0) || c.f.Ed != 0.0 || c.f.Uq != 0.0;
+    const bool timed = (c.f.trap && c.f.Vrf != 0.0) || c.f.Ed != 0.0 || c.f.Uq != 0.0 ||
+                       c.f.Uo != 0.0 || c.f.Uw != 0.0 || c.f.Eax != 0.0;
     p.fsal = !c.f.ramp && (e->cfg.time_mode == SBK_TIME_STAGE || !timed);
     return p;
 }
@@ -761,6 +777,10 @@ void sbk_default_fields(sbk_fields *f)
     f->coulomb = 0;
     f->coulomb_scale = 1.0;
     f->rq = 1.0;
+    f->ro = 1.0;
+    f->rw = 1.0;
+    f->rw_order = 2;
+    f->T_gas = 273.15;
 }
 
 int sbk_spatial_order(int n, const double *pos, int *order)
@@ -830,10 +850,24 @@ int sbk_destroy(sbk_engine *e)
     return 0;
 }
 
+// Parameters the field terms divide by, or choose a form with.
+static int check_fields(const sbk_fields *f)
+{
+    if (f->Uw != 0.0 && f->rw_order != 1 && f->rw_order != 2)
+        return fail(SBK_EINVAL, "rw_order must be 1 or 2 (got %d)", f->rw_order);
+    if ((f->Uw != 0.0 && !(f->rw > 0.0)) || (f->Uo != 0.0 && !(f->ro > 0.0)) ||
+        (f->Uq != 0.0 && !(f->rq > 0.0)) || (f->V0 != 0.0 && !(f->d0 > 0.0)))
+        return fail(SBK_EINVAL, "electrode radii rq, ro, rw and d0 must be > 0 when used");
+    if (f->K0 != 0.0 && f->p_mbar != 0.0 && !(f->T_gas > 0.0))
+        return fail(SBK_EINVAL, "T_gas must be > 0 for the buffer gas");
+    return 0;
+}
+
 int sbk_add_cloud(sbk_engine *e, int n, const double *pos, const double *vel, const double *mass,
                   const int *charge, const sbk_fields *f, double lifetime, int *cloud_id)
 {
     if (!e || !f) return fail(SBK_EINVAL, "null argument");
+    if (int rc = check_fields(f)) return rc;
     if (e->finalized) return fail(SBK_ESTATE, "clouds must be added before the first run");
     if (n < 0 || (n > 0 && (!pos || !vel || !mass || !charge)))
         return fail(SBK_EINVAL, "bad cloud arrays (n=%d)", n);
@@ -858,6 +892,7 @@ int sbk_set_fields(sbk_engine *e, int cloud, const sbk_fields *f)
 {
     if (!e || !f || cloud < 0 || cloud >= (int)e->clouds.size())
         return fail(SBK_EINVAL, "bad cloud id");
+    if (int rc = check_fields(f)) return rc;
     HostCloud &c = e->clouds[cloud];
     const bool coulomb_changed = (c.f.coulomb != 0) != (f->coulomb != 0);
     const bool scale_changed = c.f.coulomb_scale != f->coulomb_scale;

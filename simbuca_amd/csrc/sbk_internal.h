@@ -39,6 +39,13 @@ struct CloudParams {
     double Ed, fd, phid;
     double Uq_rq2, fq, phiq;  // Uq / rq^2
     double delta;
+    double fd_rate, fq_rate;  // drive frequency sweeps [Hz/s]
+    double Uo_ro4, fo, phio;  // octupole drive, Uo / ro^4
+    int rw_order, pad3;       // rotating wall: order 1 or 2, amplitude Uw / rw^order
+    double Uw_rwl, fw, phiw;
+    double Eax, fax, phiax;   // axial dipole drive
+    double B2;                // magnetic bottle
+    double inv_K;             // buffer gas: 1 / mobility (0: off)
     // layout
     int off, n, slices, pad;
     int blk0, nblk;           // this cloud's range in the ion-block table

@@ -444,9 +444,11 @@ __device__ __forceinline__ void ion_accel(const CloudParams &p, double t_life, d
         ax += qm * p.Bz * vy;
         ay -= qm * p.Bz * vx;
     }
-    if (p.Ed != 0.0) ax += qm * (p.Ed * cos(2.0 * kPi * p.fd * ts + p.phid));
+    // drive phases 2 pi (f t + rate t^2 / 2) + phi; rate = 0 adds an exact 0
+    if (p.Ed != 0.0)
+        ax += qm * (p.Ed * cos(2.0 * kPi * p.fd * ts + kPi * p.fd_rate * ts * ts + p.phid));
     if (p.Uq_rq2 != 0.0) {
-        const double cq = p.Uq_rq2 * cos(2.0 * kPi * p.fq * ts + p.phiq);
+        const double cq = p.Uq_rq2 * cos(2.0 * kPi * p.fq * ts + kPi * p.fq_rate * ts * ts + p.phiq);
         ax += qm * cq * x;
         ay -= qm * cq * y;
     }
@@ -454,6 +456,42 @@ __device__ __forceinline__ void ion_accel(const CloudParams &p, double t_life, d
         ax -= p.delta * vx;
         ay -= p.delta * vy;
         az -= p.delta * vz;
+    }
+    if (p.B2 != 0.0) {
+        // magnetic bottle B2 ((z^2 - (x^2+y^2)/2) z_hat - z (x, y, 0)); a += (q e/m) v x B
+        const double b2z = p.B2 * z;
+        const double bx = -b2z * x, by = -b2z * y;
+        const double bz = p.B2 * (z * z - 0.5 * (x * x + y * y));
+        ax += qm * (vy * bz - vz * by);
+        ay += qm * (vz * bx - vx * bz);
+        az += qm * (vx * by - vy * bx);
+    }
+    if (p.Uo_ro4 != 0.0) {
+        // octupole drive: -grad of -(Uo/(4 ro^4)) cos(psi) Re (x+iy)^4
+        const double co = p.Uo_ro4 * cos(2.0 * kPi * p.fo * ts + p.phio);
+        ax += qm * co * (x * (x * x - 3.0 * y * y));
+        ay += qm * co * (y * (y * y - 3.0 * x * x));
+    }
+    if (p.Uw_rwl != 0.0) {
+        // rotating wall of order 1 (uniform field) or 2 (quadrupole), turning at fw
+        const double psi = 2.0 * kPi * p.fw * ts + p.phiw;
+        const double ang = (p.rw_order == 1) ? psi : 2.0 * psi;
+        const double cw = p.Uw_rwl * cos(ang), sw = p.Uw_rwl * sin(ang);
+        if (p.rw_order == 1) {
+            ax += qm * cw;
+            ay += qm * sw;
+        } else {
+            ax += qm * (cw * x + sw * y);
+            ay += qm * (sw * x - cw * y);
+        }
+    }
+    if (p.Eax != 0.0) az += qm * (p.Eax * cos(2.0 * kPi * p.fax * ts + p.phiax));
+    if (p.inv_K != 0.0) {
+        // buffer gas by mobility: a -= (q e/m) v / K
+        const double gk = qm * p.inv_K;
+        ax -= gk * vx;
+        ay -= gk * vy;
+        az -= gk * vz;
     }
     ax -= Fx;
     ay -= Fy;

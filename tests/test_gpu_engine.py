@@ -93,6 +93,70 @@ def test_force_time_dependent_terms(rng):
     assert np.allclose(out, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
 
 
+def test_field_extras_bit_identical(rng):
+    # magnetic bottle and buffer gas by mobility have no cosine: bit-identical
+    amu, m, ch, pos, vel = _rand_cloud(rng, 600)
+    kw = dict(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0, B2=2.5e4, K0=1.7e-4, p_mbar=3e-3,
+              T_gas=295.0, delta=40.0)
+    sf, of = _both_fields(**kw)
+    with S.Engine(precision="fp64_exact") as e:
+        e.add_cloud(pos, vel, m, ch, sf, lifetime=1.1e-6)
+        out = e.eval_force(0)
+    assert np.array_equal(out, O.force(pos, vel, m, ch, of, 1.1e-6))
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_excitation_menu_matches_oracle(rng, order):
+    # octupole, rotating wall, axial drive and swept dipole/quadrupole drives
+    amu, m, ch, pos, vel = _rand_cloud(rng, 400)
+    kw = dict(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0, Ed=12.0, fd=9.4e2, fd_rate=3e9,
+              phid=0.3, Uq=0.2, rq=9e-3, fq=8.08e5, fq_rate=-2e10, phiq=-0.7, Uo=0.5, ro=7e-3,
+              fo=1.6e6, phio=0.2, rw_order=order, Uw=0.3, rw=6e-3, fw=2.5e5, phiw=1.1, Eax=3.0,
+              fax=1.9e5, phiax=-0.4)
+    sf, of = _both_fields(**kw)
+    with S.Engine(precision="fp64_exact") as e:
+        e.add_cloud(pos, vel, m, ch, sf, lifetime=4.3e-6)
+        out = e.eval_force(0)
+    ref = O.force(pos, vel, m, ch, of, 4.3e-6)
+    # only the device and host cos/sin may differ, by an ulp
+    assert np.allclose(out, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+
+
+def test_excitation_menu_trajectory_vs_oracle(rng):
+    # stage-time DP5 with every new term on, in the mixed engine against the
+    # FP64 oracle over 30 fixed steps
+    amu, m, ch, pos, vel = _rand_cloud(rng, 300, species=(133.0, 85.0), charges=(1,))
+    nu_c, _, nu_m, _ = synth.penning_frequencies(133.0)
+    kw = dict(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0, B2=1e4, Ed=10.0, fd=nu_m, fd_rate=1e9,
+              Uq=0.1, rq=11e-3, fq=nu_c, Uo=0.05, ro=11e-3, fo=2 * nu_c, rw_order=2, Uw=0.02,
+              rw=11e-3, fw=1e4, Eax=1.0, fax=3e5, K0=1.6e-4, p_mbar=1e-4, T_gas=300.0)
+    sf, of = _both_fields(**kw)
+    with S.Engine(precision="mixed", adaptive=False, h0=2e-9, time_mode=S.TIME_STAGE) as e:
+        e.add_cloud(pos, vel, m, ch, sf)
+        e.advance(5.9e-8)
+        gp, gv = e.state(0)
+        st = e.status(0)
+    c = O.Cloud(pos, vel, m, ch, of, h0=2e-9, adaptive=False, time_mode=O.TIME_STAGE)
+    for _ in range(st["steps"]):
+        c.step()
+    op, ov, sc = c.state()
+    assert st["steps"] == 30 and st["lifetime"] == sc["lifetime"]
+    assert np.allclose(gp, op, rtol=0, atol=1e-12 * np.abs(op).max())
+    assert np.allclose(gv, ov, rtol=0, atol=1e-8 * np.abs(ov).max())
+
+
+def test_field_parameters_are_validated(rng):
+    amu, m, ch, pos, vel = _rand_cloud(rng, 10)
+    with S.Engine(precision="fp64_exact") as e:
+        with pytest.raises(S.SimbucaError, match="rw_order"):
+            e.add_cloud(pos, vel, m, ch, S.default_fields(trap=0, Uw=1.0, rw_order=3))
+        with pytest.raises(S.SimbucaError, match="radii"):
+            e.add_cloud(pos, vel, m, ch, S.default_fields(trap=0, Uo=1.0, ro=0.0))
+        with pytest.raises(S.SimbucaError, match="T_gas"):
+            e.add_cloud(pos, vel, m, ch, S.default_fields(trap=0, K0=1e-4, p_mbar=1.0, T_gas=0.0))
+        e.add_cloud(pos, vel, m, ch, S.default_fields(trap=0, Uw=1.0, rw_order=1))
+
+
 @pytest.fixture(scope="module")
 def exact_65536():
     """FP64-exact Coulomb sum of the 65,536-ion cloud, summed in the caller's
