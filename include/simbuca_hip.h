@@ -92,7 +92,8 @@ enum {
     SBK_EHIP = -2,       /* HIP runtime error */
     SBK_ESTATE = -3,     /* call not valid in the engine's current state */
     SBK_EUNDERFLOW = -4, /* "stepsize underflow created" (bin/ode.cpp:482-483) */
-    SBK_ENOMEM = -5
+    SBK_ENOMEM = -5,
+    SBK_EIO = -6         /* a file could not be opened, read or parsed */
 };
 
 /* Coulomb arithmetic. */
@@ -270,6 +271,33 @@ int  sbk_load_tuning(sbk_engine *e, int precision, const char *table_path);
  * a multiple of the largest power of two A <= 1024 with 2A <= part size, so
  * aligned runs of 64..1024 ions are spatially compact. */
 int  sbk_spatial_order(int n, const double *pos, int *order);
+
+/* ---- Output files in the reference's formats (host only; no GPU) ----------
+ * The counterpart of IonCloud's printing (bin/ioncloud.cpp:85-257, 38-58),
+ * for a cloud read back with sbk_get_state.  Files:
+ *   <prefix>_pAll.txt (or <prefix>_p<k>.txt, k = 1..n, with
+ *   per_particle_files): "ID name t(ms) x y z(mm) vx vy vz(m/s) E(eV)" per ion
+ *   and print (IonCloud::PrintParticle);
+ *   <prefix>_cloud<k>.txt, k = rank of the species name in sorted order:
+ *   "k name N t(ms) x0 y0 z0 sx sy sz(mm) E(eV)" per print (IonCloud::PrintCloud,
+ *   whose per-species sums divide by the total ion count N and whose energy adds
+ *   vz*vy, both kept so the numbers equal the reference's).
+ * digits = 6 gives the reference's "%.6e"; 17 makes restart files that
+ * sbk_import reads back to within one ulp (positions pass through mm). */
+typedef struct sbk_writer sbk_writer;
+int  sbk_writer_open(const char *prefix, int n, const char *const *names, const double *mass_amu,
+                     int per_particle_files, int digits, sbk_writer **out);
+int  sbk_writer_particles(sbk_writer *w, double lifetime, const double *pos, const double *vel);
+int  sbk_writer_cloud(sbk_writer *w, double lifetime, const double *pos, const double *vel);
+/* IonCloud::Delete: a last print of every ion (if pos/vel are non-null), the
+ * closing block, and frees the writer. */
+int  sbk_writer_close(sbk_writer *w, double lifetime, const double *pos, const double *vel);
+/* Reads the last complete print of <prefix>_pAll.txt written as above: n ions
+ * in ID order, names[n][name_len], pos/vel [n][3] in m and m/s, lifetime in s.
+ * The reference's own importer (bin/parser.cpp:9-261) expects 14 columns its
+ * writer never writes; this reads the writer's 11. */
+int  sbk_import(const char *prefix, int capacity, int *n, char *names, int name_len,
+                double *pos, double *vel, double *lifetime);
 
 const char *sbk_last_error(void);
 /* Device name, compute-unit count and clock (MHz) of the current device. */

@@ -278,6 +278,30 @@ def ref_steps(mass_amu, charge, pos, vel, f: Fields, lifetime, h0, rtol, atol, a
     return pos_o, vel_o, dict(zip(keys, sc.tolist())), hist
 
 
+def ref_output(mass_amu, charge, pos, vel, lifetime, prints, out_dir, per_particle=False):
+    """The reference's own output files (IonCloud::PrintParticles/PrintCloud/
+    Delete): a print at `lifetime`, then `prints` times pos += 1e-7 vel,
+    vel *= 0.75, lifetime += 1e-7 and a print, then Delete().  Writes under
+    out_dir with prefix "ref"; returns {file name: text}.  Ion k is named
+    "%g" % mass_amu[k] + "u"."""
+    prefix = os.path.join(out_dir, "ref")
+    payload = _ref_input(6 if per_particle else 5, mass_amu, charge, pos, vel, fields(), lifetime,
+                         nsteps=prints)
+    fin = os.path.join(out_dir, "in.bin")
+    with open(fin, "wb") as fh:
+        fh.write(payload)
+    p = subprocess.run([REF_HARNESS, fin, prefix], capture_output=True, text=True, timeout=120,
+                       cwd=out_dir)
+    if p.returncode != 0:
+        raise RuntimeError(f"ref_harness failed: {p.stderr}")
+    out = {}
+    for name in sorted(os.listdir(out_dir)):
+        if name.startswith("ref_") and name.endswith(".txt"):
+            with open(os.path.join(out_dir, name)) as fh:
+                out[name[len("ref"):]] = fh.read()
+    return out
+
+
 def ref_time(mode, mass_amu, charge, pos, vel, f: Fields, lifetime, calls, h0=1e-9,
              rtol=1e-7, atol=1e-7, timeout=600):
     """Time `calls` reference force() (mode 3) or step() (mode 4) calls."""

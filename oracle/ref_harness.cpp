@@ -18,6 +18,12 @@
 //        2 steps -> pos[n][3] vel[n][3] scalars[8] hist[nsteps][2]
 //        3 time  -> one JSON line on stdout timing nsteps force() calls
 //        4 time  -> one JSON line on stdout timing nsteps step() calls
+//        5, 6 output -> the reference's own output files under the prefix
+//                 given as the second argument (5: <prefix>_pAll.txt, 6: one
+//                 <prefix>_p<k>.txt per ion; both with <prefix>_cloud<k>.txt):
+//                 PrintParticles + PrintCloud at lifetime, then nsteps times
+//                 pos += vel * 1e-7, vel *= 0.75, lifetime += 1e-7 and print
+//                 again, then Delete().  Ion k is named "<mass_amu>u" ("%g").
 #include "ode.h"
 #include "force.h"
 
@@ -69,6 +75,41 @@ int main(int argc, char **argv)
     rd(in, &pos[0], sizeof(double) * 3 * n);
     rd(in, &vel[0], sizeof(double) * 3 * n);
     fclose(in);
+
+    if (mode == 5 || mode == 6) {
+        // bin/ioncloud.cpp:29-33 (Create), 185 (use_particles_files), 270-280
+        // (AddParticle, which opens the files), 383-414 (pool vectors)
+        IonCloud *oc = new IonCloud;
+        oc->Create(argv[2]);
+        oc->use_particles_files(mode == 6);
+        for (int i = 0; i < n; i++) {
+            Ion ion((long double)mass_amu[i], charge[i]);
+            char name[64];
+            snprintf(name, sizeof(name), "%gu", mass_amu[i]);
+            ion.SetName(name);
+            oc->AddParticle(Particle(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], vel[3 * i],
+                                     vel[3 * i + 1], vel[3 * i + 2]),
+                            ion);
+        }
+        oc->InitializePoolVectors();
+        oc->CopyParticlesToVectors();
+        oc->lifetime = p[0];
+        oc->PrintParticles();
+        oc->PrintCloud();
+        for (int s = 0; s < nsteps; s++) {
+            for (int i = 0; i < n; i++)
+                for (int c = 0; c < 3; c++) {
+                    oc->pos[i][c] += oc->vel[i][c] * 1e-7;
+                    oc->vel[i][c] *= 0.75;
+                }
+            oc->lifetime += 1e-7;
+            oc->PrintParticles();
+            oc->PrintCloud();
+        }
+        oc->Delete();
+        fflush(stdout);
+        _exit(0);
+    }
 
     IonCloud *cloud = new IonCloud;
     _ode_vars *odev = new _ode_vars;

@@ -119,7 +119,8 @@ EXPORTS = [
     "sbk_set_fields", "sbk_set_step", "sbk_advance", "sbk_step", "sbk_attempts",
     "sbk_synchronize", "sbk_reorder", "sbk_get_state", "sbk_get_status", "sbk_get_stats", "sbk_reset_stats",
     "sbk_eval_force", "sbk_eval_coulomb", "sbk_set_launch", "sbk_time_coulomb", "sbk_autotune",
-    "sbk_load_tuning", "sbk_spatial_order", "sbk_last_error", "sbk_device_info",
+    "sbk_load_tuning", "sbk_spatial_order", "sbk_writer_open", "sbk_writer_particles",
+    "sbk_writer_cloud", "sbk_writer_close", "sbk_import", "sbk_last_error", "sbk_device_info",
 ]
 
 
@@ -170,6 +171,12 @@ def lib():
         "sbk_autotune": (c_int, [vp, c_int, ip, ip, ip, ip, ctypes.c_char_p]),
         "sbk_load_tuning": (c_int, [vp, c_int, ctypes.c_char_p]),
         "sbk_spatial_order": (c_int, [c_int, dp, ip]),
+        "sbk_writer_open": (c_int, [ctypes.c_char_p, c_int, ctypes.POINTER(ctypes.c_char_p), dp, c_int,
+                                    c_int, ctypes.POINTER(vp)]),
+        "sbk_writer_particles": (c_int, [vp, c_dbl, dp, dp]),
+        "sbk_writer_cloud": (c_int, [vp, c_dbl, dp, dp]),
+        "sbk_writer_close": (c_int, [vp, c_dbl, dp, dp]),
+        "sbk_import": (c_int, [ctypes.c_char_p, c_int, ip, ctypes.c_char_p, c_int, dp, dp, dp]),
         "sbk_last_error": (ctypes.c_char_p, []),
         "sbk_device_info": (c_int, [ctypes.c_char_p, c_int, ip, ip]),
     }
@@ -215,6 +222,65 @@ def spatial_order(pos):
     order = np.empty(pos.shape[0], dtype=np.int32)
     _check(lib().sbk_spatial_order(pos.shape[0], _dp(pos), _ip(order)))
     return order
+
+
+class Writer:
+    """The reference's output files (IonCloud printing) for one cloud: see
+    sbk_writer_open.  Host only."""
+
+    def __init__(self, prefix, names, mass_amu, per_particle_files=False, digits=6):
+        names = [str(x) for x in names]
+        mass = np.ascontiguousarray(mass_amu, dtype=np.float64).ravel()
+        if len(names) != mass.size:
+            raise ValueError("one name and one mass per ion")
+        self.n = mass.size
+        arr = (ctypes.c_char_p * max(1, self.n))(*[x.encode() for x in names])
+        h = ctypes.c_void_p()
+        _check(lib().sbk_writer_open(str(prefix).encode(), self.n, arr, _dp(mass),
+                                     1 if per_particle_files else 0, int(digits), ctypes.byref(h)))
+        self._h = h
+
+    def _state(self, pos, vel):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        if pos.shape[0] != self.n or vel.shape[0] != self.n:
+            raise ValueError("state does not match the writer's ions")
+        return pos, vel
+
+    def particles(self, lifetime, pos, vel):
+        pos, vel = self._state(pos, vel)
+        _check(lib().sbk_writer_particles(self._h, float(lifetime), _dp(pos), _dp(vel)))
+
+    def cloud(self, lifetime, pos, vel):
+        pos, vel = self._state(pos, vel)
+        _check(lib().sbk_writer_cloud(self._h, float(lifetime), _dp(pos), _dp(vel)))
+
+    def close(self, lifetime=0.0, pos=None, vel=None):
+        if not self._h:
+            return
+        h, self._h = self._h, None
+        if pos is None:
+            _check(lib().sbk_writer_close(h, float(lifetime), None, None))
+        else:
+            pos, vel = self._state(pos, vel)
+            _check(lib().sbk_writer_close(h, float(lifetime), _dp(pos), _dp(vel)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().sbk_writer_close(self._h, 0.0, None, None)
+
+
+def import_snapshot(prefix, capacity=1 << 20, name_len=64):
+    """Last complete print of <prefix>_pAll.txt: (names, pos [m], vel [m/s], lifetime [s])."""
+    n = ctypes.c_int()
+    names = ctypes.create_string_buffer(capacity * name_len)
+    pos = np.zeros((capacity, 3)); vel = np.zeros((capacity, 3)); t = ctypes.c_double()
+    _check(lib().sbk_import(str(prefix).encode(), capacity, ctypes.byref(n), names, name_len,
+                            _dp(pos), _dp(vel), ctypes.byref(t)))
+    k = n.value
+    raw = names.raw
+    out = [raw[i * name_len:(i + 1) * name_len].split(b"\0", 1)[0].decode() for i in range(k)]
+    return out, pos[:k].copy(), vel[:k].copy(), t.value
 
 
 def device_info():

@@ -34,6 +34,16 @@ const double kKe = 8.987551787E9;
 
 thread_local std::string g_err;
 
+}  // namespace
+
+int sbk::record_error(int code, const char *msg)
+{
+    g_err = msg;
+    return code;
+}
+
+namespace {
+
 int fail(int code, const char *fmt, ...)
 {
     char buf[512];

@@ -112,6 +112,9 @@ struct EngineDev {
     double rtol, atol;
 };
 
+// sets the message sbk_last_error() returns; returns code (sbk_engine.cpp)
+int record_error(int code, const char *msg);
+
 // host-side launchers (sbk_kernels.hip)
 enum KernelPrecision { KP_FP64_EXACT = 0, KP_FP64 = 1, KP_MIXED = 2, KP_MIXED_KAHAN = 3 };
 

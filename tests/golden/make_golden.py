@@ -3,9 +3,10 @@ CPU force()/step() (oracle/_ref/ref_harness, compiled from the reference
 sources by `make -C oracle ref`).
 
 Each fixture holds the inputs and the reference's outputs as plain arrays
-(numpy .npz, no pickles).  Run from the repository root:
+(numpy .npz, no pickles); the output-file fixtures (output_n7/) are the text
+files the reference's IonCloud writes.  Run from the repository root:
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [force] [steps] [output]
 """
 from __future__ import annotations
 
@@ -59,9 +60,38 @@ def steps_case(name, seed, n, sigma, f, h0, rtol, atol, adaptive, nsteps, lifeti
     print(name, n, nsteps, sc)
 
 
-def main():
+def output_case(name, seed, n, species, prints, lifetime):
+    """The reference's own output files (IonCloud::PrintParticles, PrintCloud,
+    Delete) for a small cloud, in both file layouts, with the inputs."""
+    import shutil
+    import tempfile
+    rng = np.random.Generator(np.random.PCG64(seed))
+    amu = rng.choice(np.array(species, dtype=np.float64), n)
+    ch = rng.choice(np.array([1, 1, 2], dtype=np.int32), n).astype(np.int32)
+    pos = rng.normal(0.0, 1e-3, (n, 3))
+    vel = rng.normal(0.0, 137.0, (n, 3))
+    base = os.path.join(OUT, name)
+    np.savez(base + ".npz", mass_amu=amu, charge=ch, pos=pos, vel=vel,
+             lifetime=np.float64(lifetime), prints=np.int32(prints))
+    for layout, per_particle in (("pAll", False), ("per_particle", True)):
+        d = os.path.join(base, layout)
+        shutil.rmtree(d, ignore_errors=True)
+        os.makedirs(d)
+        with tempfile.TemporaryDirectory() as tmp:
+            files = oracle.ref_output(amu, ch, pos, vel, lifetime, prints, tmp,
+                                      per_particle=per_particle)
+        for fname, text in files.items():
+            with open(os.path.join(d, "ref" + fname), "w") as fh:
+                fh.write(text)
+
+
+def main(which=("force", "steps", "output")):
     if not oracle.ref_available():
         raise SystemExit("oracle/_ref/ref_harness is not built (make -C oracle ref)")
+    if "output" in which:
+        output_case("output_n7", 15, 7, [133.0, 85.0, 39.0], 2, 2.5e-6)
+    if "force" not in which and "steps" not in which:
+        return
     # force(): Paul-trap RF+DC at t != 0, TOF kick, Coulomb; mixed species and charges
     f = oracle.fields(trap=1, Vrf=50.0, Vdc=5.0, kappa=0.001, r0=0.04, z0=0.04, freq_rf=50e5,
                       tof=1, E_kick=3.0, coulomb=1, coulomb_scale=1.0)
@@ -81,4 +111,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]) or ("force", "steps", "output"))
