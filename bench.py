@@ -288,13 +288,15 @@ def _device_copy(nbytes, reps):
 
 
 def kernels_only(S, w, shape=None):
-    """For counter runs: 5 mixed and 2 FP64 Coulomb launches (automatic or
-    given launch shape), then 3 device copies of 1 GiB whose traffic is known."""
-    c = w.clouds[0]
-    e = S.Engine(precision="mixed")
-    e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
-    for p, reps in (("mixed", 5), ("fp64", 2)):
-        e.set_launch(*(shape or tuned_shape(S, w, p) or (4, 4, 0)))
+    """For counter runs: 5 mixed and 2 FP64 Coulomb launches of the workload's
+    clouds (automatic or given launch shape), then 3 device copies of 1 GiB
+    whose traffic is known."""
+    mixed = "mixed_kahan" if w.precision == "mixed_kahan" else "mixed"
+    e = S.Engine(precision=mixed)
+    for c in w.clouds:
+        e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
+    for p, reps in ((mixed, 5), ("fp64", 2)):
+        e.set_launch(*(shape or tuned_shape(S, w, p) or (0, 4, 0)))
         log(f"{p} coulomb {e.time_coulomb(p, reps):.4f} ms/launch")
     e.close()
     _device_copy(1 << 30, 3)
@@ -318,6 +320,11 @@ def other_workload(S, w, args, precision):
                   "pair_interactions_per_s": st["pair_interactions"] / dt,
                   "ion_steps_per_s": st["ion_steps"] / dt,
                   "coulomb_ms_per_eval": st["coulomb_ms"] / max(1.0, st["pair_interactions"] / res["pairs_per_attempt"] * 7.0)}
+        # the Coulomb kernel against its vector peak (FP32 for mixed, FP64 for fp64)
+        peak = PEAK_FP64_TFLOPS if p == "fp64" else PEAK_FP32_TFLOPS
+        tf = FLOP_PER_PAIR * res["pairs_per_attempt"] / 7.0 / (res[p]["coulomb_ms_per_eval"] * 1e-3) / 1e12
+        res[p]["roofline"] = {"bound": "valu", "achieved": tf, "peak": peak, "unit": "TFLOP/s",
+                              "frac": tf / peak}
         e.close()
         log(f"{p}: {res[p]}")
         if args.to_end:
