@@ -144,6 +144,14 @@ typedef struct sbk_fields {
     double K0, p_mbar, T_gas;   /* buffer gas by reduced ion mobility K0 [m^2/(V s)] at pressure p_mbar [mbar]
                                    and temperature T_gas [K]: a -= (q e/m) v / K, K = K0 (1013.25/p_mbar)(T_gas/273.15);
                                    off when K0 or p_mbar is 0 */
+    /* SWIFT: a sampled waveform U(t) [V], samples at t0 + k dt (k < swift_n),
+     * linearly interpolated and 0 outside them; applied as a dipole along x,
+     * E = (U/swift_r) (1, 0, 0) (swift_kind 1), or as a quadrupole,
+     * E = (U/swift_r^2) (x, -y, 0) (swift_kind 2).  The engine copies the
+     * samples when the fields are set; off when swift_n is 0. */
+    const double *swift;
+    int    swift_n, swift_kind;
+    double swift_t0, swift_dt, swift_r;
 } sbk_fields;
 
 typedef struct sbk_config {

@@ -56,6 +56,9 @@ class Fields(ctypes.Structure):
         ("fd_rate", ctypes.c_double), ("fq_rate", ctypes.c_double),
         ("B2", ctypes.c_double),
         ("K0", ctypes.c_double), ("p_mbar", ctypes.c_double), ("T_gas", ctypes.c_double),
+        ("swift", ctypes.POINTER(ctypes.c_double)), ("swift_n", ctypes.c_int),
+        ("swift_kind", ctypes.c_int), ("swift_t0", ctypes.c_double), ("swift_dt", ctypes.c_double),
+        ("swift_r", ctypes.c_double),
     ]
 
 
@@ -74,6 +77,19 @@ def fields(**kw) -> Fields:
             raise KeyError(k)
         setattr(f, k, v)
     return f
+
+def set_swift(f, samples, t0, dt, r, kind=1):
+    """Attaches a SWIFT waveform to fields f (samples in V at t0 + k dt;
+    kind 1: dipole along x, E = U/r; kind 2: quadrupole (U/r^2)(x, -y, 0)).
+    The fields object keeps the sample array alive."""
+    a = np.ascontiguousarray(samples, dtype=np.float64).ravel()
+    f._swift_keep = a
+    f.swift = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    f.swift_n = int(a.size)
+    f.swift_kind = int(kind)
+    f.swift_t0, f.swift_dt, f.swift_r = float(t0), float(dt), float(r)
+    return f
+
 
 
 _lib = None
@@ -195,6 +211,7 @@ class Cloud:
         self._h = lib().orc_create(self.n, _dp(pos), _dp(vel), _dp(mass), _ip(charge),
                                    ctypes.byref(f), lifetime, h0, rtol, atol,
                                    1 if adaptive else 0, time_mode)
+        self._fields = f       # the C copy points at its SWIFT samples
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -203,6 +220,7 @@ class Cloud:
 
     def set_fields(self, f: Fields):
         lib().orc_set_fields(self._h, ctypes.byref(f))
+        self._fields = f
 
     def attempt(self):
         return lib().orc_dp5_attempt(self._h)

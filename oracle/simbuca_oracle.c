@@ -64,6 +64,20 @@ double orc_mass_kg(double mass_amu)
     return (double)m;
 }
 
+/* SWIFT amplitude at t: samples at t0 + k dt, linear interpolation, 0 outside */
+static double swift_amp(const orc_fields *f, double t)
+{
+    double s = (t - f->swift_t0) / f->swift_dt;
+    if (!(s >= 0.0) || s > (double)(f->swift_n - 1))
+        return 0.0;
+    int i = (int)s;
+    if (i > f->swift_n - 2)
+        i = f->swift_n - 2;
+    double w = s - (double)i;
+    double a = f->swift[i], b = f->swift[i + 1];
+    return a + w * (b - a);
+}
+
 /* bin/force.cpp:34-82 (trap/TOF) followed by the extension terms and then
  * bin/force.cpp:147-175 (Coulomb).  derivs is an acceleration. */
 void orc_force(int n, const double *pos_, const double *vel_, const double *mass,
@@ -100,6 +114,7 @@ void orc_force(int n, const double *pos_, const double *vel_, const double *mass
     double cax = (f->Eax != 0.0) ? f->Eax * cos(2.0 * pi * f->fax * t + f->phiax) : 0.0;
     double invK = (f->K0 != 0.0 && f->p_mbar != 0.0)
                       ? (f->p_mbar / 1013.25) * (273.15 / f->T_gas) / f->K0 : 0.0;
+    double uswift = (f->swift_n > 0) ? swift_amp(f, t) : 0.0;
 
     for (int j_ = 0; j_ < n; j_++) {
         double x = pos[j_][0], y = pos[j_][1], z = pos[j_][2];
@@ -164,6 +179,15 @@ void orc_force(int n, const double *pos_, const double *vel_, const double *mass
             derivs[j_][0] -= g * vx;
             derivs[j_][1] -= g * vy;
             derivs[j_][2] -= g * vz;
+        }
+        if (f->swift_n > 0) {
+            if (f->swift_kind == 1) {
+                derivs[j_][0] += qDmassa * (uswift / f->swift_r);
+            } else {
+                double cs = uswift / (f->swift_r * f->swift_r);
+                derivs[j_][0] += qDmassa * cs * x;
+                derivs[j_][1] -= qDmassa * cs * y;
+            }
         }
     }
 

@@ -63,6 +63,11 @@ typedef struct orc_fields {
     double B2;
     /* buffer gas by reduced mobility K0 at p_mbar and T_gas */
     double K0, p_mbar, T_gas;
+    /* SWIFT waveform: samples at swift_t0 + k swift_dt, linear interpolation,
+     * dipole (kind 1, E = U/r along x) or quadrupole (kind 2, (U/r^2)(x, -y, 0)) */
+    const double *swift;
+    int    swift_n, swift_kind;
+    double swift_t0, swift_dt, swift_r;
 } orc_fields;
 
 enum { ORC_TIME_FROZEN = 0, ORC_TIME_STAGE = 1 };

@@ -67,6 +67,9 @@ class Fields(ctypes.Structure):
         ("fd_rate", ctypes.c_double), ("fq_rate", ctypes.c_double),
         ("B2", ctypes.c_double),
         ("K0", ctypes.c_double), ("p_mbar", ctypes.c_double), ("T_gas", ctypes.c_double),
+        ("swift", ctypes.POINTER(ctypes.c_double)), ("swift_n", ctypes.c_int),
+        ("swift_kind", ctypes.c_int), ("swift_t0", ctypes.c_double), ("swift_dt", ctypes.c_double),
+        ("swift_r", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -213,6 +216,19 @@ def default_fields(**kw) -> Fields:
             raise KeyError(k)
         setattr(f, k, v)
     return f
+
+def set_swift(f, samples, t0, dt, r, kind=1):
+    """Attaches a SWIFT waveform to fields f (samples in V at t0 + k dt;
+    kind 1: dipole along x, E = U/r; kind 2: quadrupole (U/r^2)(x, -y, 0)).
+    The fields object keeps the sample array alive."""
+    a = np.ascontiguousarray(samples, dtype=np.float64).ravel()
+    f._swift_keep = a
+    f.swift = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    f.swift_n = int(a.size)
+    f.swift_kind = int(kind)
+    f.swift_t0, f.swift_dt, f.swift_r = float(t0), float(dt), float(r)
+    return f
+
 
 
 def spatial_order(pos):

@@ -70,6 +70,7 @@ struct HostCloud {
     std::vector<double> pos, vel, mass;   // in device (slot) order after finalize
     std::vector<int> charge;
     std::vector<int> perm;                // slot -> caller's index (empty: identity)
+    std::vector<double> swift;            // SWIFT samples, copied from the fields
 };
 
 struct ItemList {
@@ -101,6 +102,9 @@ struct sbk_engine {
     double *d_cp = nullptr;
     int cp_slices = 0;
     sbk::CloudParams *d_params = nullptr;
+    double *d_swift = nullptr;             // every cloud's SWIFT samples
+    size_t swift_cap = 0;
+    std::vector<size_t> swift_off;
     sbk::CloudState *d_state = nullptr;
     sbk::CloudState *h_state = nullptr;   // pinned mirror
     double *d_kc = nullptr, *d_qd = nullptr;
@@ -174,6 +178,12 @@ sbk::CloudParams make_params(const sbk_fields &f, int off, int n)
     p.B2 = f.B2;
     p.inv_K = (f.K0 != 0.0 && f.p_mbar != 0.0) ? (f.p_mbar / 1013.25) * (273.15 / f.T_gas) / f.K0
                                                 : 0.0;
+    p.swift = nullptr;   // set by cloud_params
+    p.swift_n = f.swift_n;
+    p.swift_kind = f.swift_kind;
+    p.swift_t0 = f.swift_t0;
+    p.swift_dt = f.swift_dt;
+    p.swift_r = f.swift_r;
     p.off = off;
     p.n = n;
     p.slices = 1;
@@ -400,9 +410,31 @@ sbk::CloudParams cloud_params(const sbk_engine *e, size_t ci)
     // reference's frozen time it is exact only if no term depends on time; the
     // trap ramp is evaluated at the step's start time, so it never is.
     const bool timed = (c.f.trap && c.f.Vrf != 0.0) || c.f.Ed != 0.0 || c.f.Uq != 0.0 ||
-                       c.f.Uo != 0.0 || c.f.Uw != 0.0 || c.f.Eax != 0.0;
+                       c.f.Uo != 0.0 || c.f.Uw != 0.0 || c.f.Eax != 0.0 || c.f.swift_n > 0;
+    if (c.f.swift_n > 0 && e->d_swift) p.swift = e->d_swift + e->swift_off[ci];
     p.fsal = !c.f.ramp && (e->cfg.time_mode == SBK_TIME_STAGE || !timed);
     return p;
+}
+
+// Concatenates every cloud's SWIFT samples into one device buffer (grown as
+// needed; the stream must be idle, since the old buffer may be freed).
+int upload_swift(sbk_engine *e)
+{
+    std::vector<double> all;
+    e->swift_off.assign(e->clouds.size(), 0);
+    for (size_t ci = 0; ci < e->clouds.size(); ++ci) {
+        e->swift_off[ci] = all.size();
+        all.insert(all.end(), e->clouds[ci].swift.begin(), e->clouds[ci].swift.end());
+    }
+    if (all.empty()) return 0;
+    if (all.size() > e->swift_cap) {
+        if (e->d_swift) HIP_TRY(hipFree(e->d_swift));
+        e->d_swift = nullptr;
+        HIP_TRY(hipMalloc(&e->d_swift, sizeof(double) * all.size()));
+        e->swift_cap = all.size();
+    }
+    HIP_TRY(hipMemcpy(e->d_swift, all.data(), sizeof(double) * all.size(), hipMemcpyHostToDevice));
+    return 0;
 }
 
 int upload_params(sbk_engine *e)
@@ -411,6 +443,7 @@ int upload_params(sbk_engine *e)
     for (size_t ci = 0; ci < e->clouds.size(); ++ci) ps.push_back(cloud_params(e, ci));
     HIP_TRY(hipMemcpyAsync(e->d_params, ps.data(), sizeof(sbk::CloudParams) * ps.size(),
                            hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));   // ps is pageable and goes out of scope
     return 0;
 }
 
@@ -502,6 +535,7 @@ int finalize(sbk_engine *e)
     HIP_TRY(hipMemcpyAsync(e->d_qf, qf.data(), sizeof(float) * S, hipMemcpyHostToDevice,
                            e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));   // pageable staging goes out of scope
+    if (int rc = upload_swift(e)) return rc;
     std::vector<sbk::CloudState> st(e->clouds.size());
     for (size_t ci = 0; ci < e->clouds.size(); ++ci) {
         sbk::CloudState s{};
@@ -854,6 +888,7 @@ int sbk_destroy(sbk_engine *e)
     (void)hipFree(e->d_params);
     (void)hipFree(e->d_state);
     (void)hipFree(e->d_src);
+    (void)hipFree(e->d_swift);
     if (e->h_state) (void)hipHostFree(e->h_state);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -870,6 +905,11 @@ static int check_fields(const sbk_fields *f)
         return fail(SBK_EINVAL, "electrode radii rq, ro, rw and d0 must be > 0 when used");
     if (f->K0 != 0.0 && f->p_mbar != 0.0 && !(f->T_gas > 0.0))
         return fail(SBK_EINVAL, "T_gas must be > 0 for the buffer gas");
+    if (f->swift_n < 0 || (f->swift_n > 0 && (!f->swift || f->swift_n < 2 || !(f->swift_dt > 0.0) ||
+                                             !(f->swift_r > 0.0) ||
+                                             (f->swift_kind != 1 && f->swift_kind != 2))))
+        return fail(SBK_EINVAL, "SWIFT needs >= 2 samples, swift_dt > 0, swift_r > 0 and "
+                                "swift_kind 1 or 2");
     return 0;
 }
 
@@ -892,6 +932,8 @@ int sbk_add_cloud(sbk_engine *e, int n, const double *pos, const double *vel, co
     c.charge.assign(charge, charge + n);
     for (int i = 0; i < n; ++i)
         if (!(c.mass[i] > 0.0)) return fail(SBK_EINVAL, "ion %d has mass <= 0", i);
+    if (f->swift_n > 0) c.swift.assign(f->swift, f->swift + f->swift_n);
+    c.f.swift = nullptr;   // the samples live in c.swift from here on
     e->clouds.push_back(std::move(c));
     e->ntot += n;
     if (cloud_id) *cloud_id = (int)e->clouds.size() - 1;
@@ -907,10 +949,14 @@ int sbk_set_fields(sbk_engine *e, int cloud, const sbk_fields *f)
     const bool coulomb_changed = (c.f.coulomb != 0) != (f->coulomb != 0);
     const bool scale_changed = c.f.coulomb_scale != f->coulomb_scale;
     c.f = *f;
+    c.swift.assign(f->swift, f->swift + (f->swift_n > 0 ? f->swift_n : 0));
+    c.f.swift = nullptr;
     if (!e->finalized) return 0;
     HIP_TRY(hipSetDevice(e->device));
-    sbk::CloudParams p = cloud_params(e, cloud);
-    HIP_TRY(hipMemcpyAsync(e->d_params + cloud, &p, sizeof(p), hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));   // the sample buffer may be reallocated
+    if (int rc = upload_swift(e)) return rc;
+    // every cloud's parameters: the samples of the others may have moved
+    if (int rc = upload_params(e)) return rc;
     // the stored stage-1 derivative was computed with the old fields
     int rc = read_states(e);
     if (rc) return rc;
@@ -923,6 +969,7 @@ int sbk_set_fields(sbk_engine *e, int cloud, const sbk_fields *f)
         ion_constants(c, qm.data(), kc.data(), qd.data(), qf.data());
         HIP_TRY(hipMemcpyAsync(e->d_kc + c.off, kc.data(), sizeof(double) * c.n,
                                hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));   // kc goes out of scope
     }
     if (coulomb_changed) free_lists(e);
     HIP_TRY(hipStreamSynchronize(e->stream));

@@ -46,6 +46,9 @@ struct CloudParams {
     double Eax, fax, phiax;   // axial dipole drive
     double B2;                // magnetic bottle
     double inv_K;             // buffer gas: 1 / mobility (0: off)
+    const double *swift;      // SWIFT samples on the device (swift_n > 0)
+    int swift_n, swift_kind;
+    double swift_t0, swift_dt, swift_r;
     // layout
     int off, n, slices, pad;
     int blk0, nblk;           // this cloud's range in the ion-block table

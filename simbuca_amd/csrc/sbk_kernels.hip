@@ -410,6 +410,18 @@ __device__ __forceinline__ double combo(double *const (*K)[3], int comp, long lo
     return acc;
 }
 
+// SWIFT amplitude at t: samples at t0 + k dt, linear interpolation, 0 outside
+__device__ __forceinline__ double swift_amp(const CloudParams &p, double t)
+{
+    const double s = (t - p.swift_t0) / p.swift_dt;
+    if (!(s >= 0.0) || s > (double)(p.swift_n - 1)) return 0.0;
+    int i = (int)s;
+    if (i > p.swift_n - 2) i = p.swift_n - 2;
+    const double w = s - (double)i;
+    const double a = p.swift[i], b = p.swift[i + 1];
+    return a + w * (b - a);
+}
+
 // force() at one ion (bin/force.cpp:58-105 order), then the extension terms,
 // then the Coulomb subtraction.  t_life is IonCloud::lifetime (drives the
 // trap ramp, bin/ion_fly.cpp:235-239); ts is the field evaluation time.
@@ -492,6 +504,17 @@ __device__ __forceinline__ void ion_accel(const CloudParams &p, double t_life, d
         ax -= gk * vx;
         ay -= gk * vy;
         az -= gk * vz;
+    }
+    if (p.swift_n > 0) {
+        // SWIFT: sampled waveform as a dipole along x or a quadrupole
+        const double u = swift_amp(p, ts);
+        if (p.swift_kind == 1) {
+            ax += qm * (u / p.swift_r);
+        } else {
+            const double cs = u / (p.swift_r * p.swift_r);
+            ax += qm * cs * x;
+            ay -= qm * cs * y;
+        }
     }
     ax -= Fx;
     ay -= Fy;

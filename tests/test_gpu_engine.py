@@ -145,6 +145,52 @@ def test_excitation_menu_trajectory_vs_oracle(rng):
     assert np.allclose(gv, ov, rtol=0, atol=1e-8 * np.abs(ov).max())
 
 
+@pytest.mark.parametrize("kind", [1, 2])
+def test_swift_bit_identical(rng, kind):
+    # linear interpolation has no transcendental: bit-identical to the oracle,
+    # inside and outside the sampled window
+    amu, m, ch, pos, vel = _rand_cloud(rng, 500)
+    samples = rng.normal(0, 0.3, 257)
+    sf = S.set_swift(S.default_fields(trap=0, coulomb=1, Bz=7.0, V0=10.0, d0=11e-3),
+                     samples, 2e-7, 3e-9, 7e-3, kind)
+    of = O.set_swift(O.fields(trap=0, coulomb=1, Bz=7.0, V0=10.0, d0=11e-3),
+                     samples, 2e-7, 3e-9, 7e-3, kind)
+    for t in (1e-7, 2e-7, 4.123e-7, 9.68e-7, 2e-7 + 256 * 3e-9, 1.2e-6):
+        with S.Engine(precision="fp64_exact") as e:
+            e.add_cloud(pos, vel, m, ch, sf, lifetime=t)
+            out = e.eval_force(0)
+        assert np.array_equal(out, O.force(pos, vel, m, ch, of, t)), t
+
+
+def test_swift_set_fields_moves_samples(rng):
+    # two clouds; replacing cloud 0's waveform with a longer one reallocates the
+    # device buffer and moves cloud 1's samples: both must follow the oracle
+    clouds = [_rand_cloud(rng, n, species=(133.0,), charges=(1,)) for n in (120, 90)]
+    w0, w1, w0b = rng.normal(0, 1, 50), rng.normal(0, 1, 80), rng.normal(0, 1, 400)
+    base = dict(trap=0, coulomb=1, Bz=7.0, V0=10.0, d0=11e-3)
+    sfs = [S.set_swift(S.default_fields(**base), w0, 0.0, 2e-9, 5e-3, 1),
+           S.set_swift(S.default_fields(**base), w1, 0.0, 2e-9, 5e-3, 2)]
+    ofs = [O.set_swift(O.fields(**base), w0b, 0.0, 2e-9, 5e-3, 1),
+           O.set_swift(O.fields(**base), w1, 0.0, 2e-9, 5e-3, 2)]
+    with S.Engine(precision="fp64_exact", adaptive=False, h0=1e-9, time_mode=S.TIME_STAGE) as e:
+        for (_, m, ch, pos, vel), sf in zip(clouds, sfs):
+            e.add_cloud(pos, vel, m, ch, sf)
+        e.step()
+        e.set_fields(0, S.set_swift(S.default_fields(**base), w0b, 0.0, 2e-9, 5e-3, 1))
+        for _ in range(5):
+            e.step()
+        got = [e.state(i) for i in range(2)]
+    for ci, ((_, m, ch, pos, vel), of) in enumerate(zip(clouds, ofs)):
+        first = O.set_swift(O.fields(**base), w0, 0.0, 2e-9, 5e-3, 1) if ci == 0 else of
+        c = O.Cloud(pos, vel, m, ch, first, h0=1e-9, adaptive=False, time_mode=O.TIME_STAGE)
+        c.step()
+        c.set_fields(of)
+        for _ in range(5):
+            c.step()
+        op, ov, _ = c.state()
+        assert np.array_equal(got[ci][0], op) and np.array_equal(got[ci][1], ov), ci
+
+
 def test_field_parameters_are_validated(rng):
     amu, m, ch, pos, vel = _rand_cloud(rng, 10)
     with S.Engine(precision="fp64_exact") as e:
@@ -154,6 +200,10 @@ def test_field_parameters_are_validated(rng):
             e.add_cloud(pos, vel, m, ch, S.default_fields(trap=0, Uo=1.0, ro=0.0))
         with pytest.raises(S.SimbucaError, match="T_gas"):
             e.add_cloud(pos, vel, m, ch, S.default_fields(trap=0, K0=1e-4, p_mbar=1.0, T_gas=0.0))
+        with pytest.raises(S.SimbucaError, match="SWIFT"):
+            e.add_cloud(pos, vel, m, ch, S.set_swift(S.default_fields(trap=0), [1.0], 0.0, 1e-9, 1.0))
+        with pytest.raises(S.SimbucaError, match="SWIFT"):
+            e.add_cloud(pos, vel, m, ch, S.set_swift(S.default_fields(trap=0), [1.0, 2.0], 0.0, 0.0, 1.0))
         e.add_cloud(pos, vel, m, ch, S.default_fields(trap=0, Uw=1.0, rw_order=1))
 
 

@@ -267,3 +267,62 @@ def test_buffer_gas_mobility_closed_form():
     f0 = O.fields(trap=0, K0=K0, p_mbar=1013.25, T_gas=273.15)
     a = O.force(pos, vel, m, ch, f0, 0.0)
     assert np.allclose(a, -(ch * E / m / K0)[:, None] * vel, rtol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# SWIFT: sampled waveform as a dipole or a quadrupole
+# ---------------------------------------------------------------------------
+
+def test_swift_constant_waveform_is_a_static_field(rng):
+    # a constant U gives E = U/r along x: the TOF kick's form (bin/force.cpp:77-80)
+    pos = rng.normal(0, 1e-3, (9, 3)); vel = rng.normal(0, 100.0, (9, 3))
+    m = O.mass_kg(np.full(9, 133.0)); ch = np.ones(9, dtype=np.int32)
+    f = O.set_swift(O.fields(trap=0), np.full(17, 2.0), 1e-6, 1e-7, 4e-3, kind=1)
+    a = O.force(pos, vel, m, ch, f, 1.73e-6)
+    ref = O.force(pos, vel, m, ch, O.fields(trap=0, tof=1, E_kick=2.0 / 4e-3), 1.73e-6)
+    assert np.array_equal(a, ref)
+    # outside the sampled window there is no field at all
+    assert np.array_equal(O.force(pos, vel, m, ch, f, 0.9e-6), np.zeros((9, 3)))
+    assert np.array_equal(O.force(pos, vel, m, ch, f, 1e-6 + 16.5e-7), np.zeros((9, 3)))
+
+
+def test_swift_interpolation_and_quadrupole_form(rng):
+    # a sampled cosine equals the quadrupole drive to the interpolation error
+    # (w dt)^2 / 8, and a linear waveform is interpolated exactly
+    Uq, rq, fq = 0.4, 9e-3, 3e5
+    dt = 1e-8
+    t = np.arange(0, 401) * dt
+    f = O.set_swift(O.fields(trap=0), Uq * np.cos(2 * math.pi * fq * t), 0.0, dt, rq, kind=2)
+    g = O.fields(trap=0, Uq=Uq, rq=rq, fq=fq)
+    pos = rng.normal(0, 1e-3, (6, 3)); vel = np.zeros((6, 3))
+    m = O.mass_kg(np.full(6, 85.0)); ch = np.ones(6, dtype=np.int32)
+    # |U_interp - U| <= (w dt)^2 / 8 * Uq for a linear interpolant of Uq cos(w t)
+    amp = (E / m[0]) * Uq / rq ** 2 * np.abs(pos[:, :2]).max()
+    bound = (2 * math.pi * fq * dt) ** 2 / 8 * amp
+    for tt in rng.uniform(0, 400 * dt, 7):
+        a = O.force(pos, vel, m, ch, f, tt)
+        b = O.force(pos, vel, m, ch, g, tt)
+        assert np.abs(a - b).max() <= 1.01 * bound
+        assert np.all(a[:, 2] == 0.0)
+        # and the field has the quadrupole's form: a_x / x = -a_y / y
+        assert np.allclose(a[:, 0] / pos[:, 0], -a[:, 1] / pos[:, 1], rtol=1e-12)
+    lin = O.set_swift(O.fields(trap=0), 3.0 + 2.0 * np.arange(5), 0.0, 1.0, 1.0, kind=1)
+    E_at = lambda tt: O.force(np.zeros((1, 3)), np.zeros((1, 3)), m[:1], ch[:1], lin, tt)[0, 0] / (E / m[0])
+    for tt in (0.0, 0.25, 1.5, 2.75, 4.0):
+        assert E_at(tt) == pytest.approx(3.0 + 2.0 * tt, rel=1e-15)
+
+
+def test_swift_driven_free_ion():
+    # a sampled cos(w t) drive: the free ion follows A (1 - cos w t) / w^2 up to
+    # the interpolation error of the drive
+    p, v, m, ch = _single([0, 0, 0], [0, 0, 0])
+    w = 2 * math.pi * 1e5
+    dt = 2e-9
+    t = np.arange(0, 20001) * dt
+    f = O.set_swift(O.fields(trap=0), 6.0 * np.cos(w * t), 0.0, dt, 1.0, kind=1)
+    A = E * 6.0 / m[0]
+    c = O.Cloud(p, v, m, ch, f, h0=1e-8, rtol=1e-11, atol=1e-16, time_mode=O.TIME_STAGE)
+    c.advance(3.3e-5)
+    pos, vel, sc = c.state()
+    tt = sc["lifetime"]
+    assert pos[0, 0] == pytest.approx(A * (1 - math.cos(w * tt)) / w ** 2, rel=1e-5)
