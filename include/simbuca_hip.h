@@ -179,6 +179,7 @@ typedef struct sbk_cloud_status {
     int    n;
     int    slices;      /* source slices of the Coulomb work split */
     long long coulomb_evals;  /* Coulomb evaluations done (7 or 6 per attempt) */
+    double t_prev;      /* start time of the last accepted step (lifetime before it) */
 } sbk_cloud_status;
 
 typedef struct sbk_stats {
@@ -217,6 +218,9 @@ int  sbk_set_step(sbk_engine *e, int cloud, double h);
  * or no time-dependent field; never with the trap ramp).  Results are
  * bit-identical either way; those attempts cost 6 evaluations. */
 int  sbk_advance(sbk_engine *e, double duration, long long max_attempts);
+/* As sbk_advance, to the absolute time t_end: every cloud steps while its
+ * lifetime < t_end (the loop condition of move_particles). */
+int  sbk_advance_until(sbk_engine *e, double t_end, long long max_attempts);
 /* One accepted step for every cloud (step(), bin/ode.cpp:459-494). */
 int  sbk_step(sbk_engine *e);
 /* Enqueues `count` attempts for the active clouds without waiting. */
@@ -306,6 +310,50 @@ int  sbk_writer_close(sbk_writer *w, double lifetime, const double *pos, const d
  * writer never writes; this reads the writer's 11. */
 int  sbk_import(const char *prefix, int capacity, int *n, char *names, int name_len,
                 double *pos, double *vel, double *lifetime);
+
+/* ---- The reference's simulation set-up (host only, except sbk_run_ini) ----
+ * sbk_init_cloud: InitCloud (bin/initialization.cpp:117-206).  n ions in the
+ * ellipsoid offset + semiaxis * (unit ball), velocities of Maxwell-Boltzmann
+ * speed around energy_eV[0] with isotropic directions, drawn from MT19937
+ * seeded with `seed` (bin/mtrand.h:96-104) in the reference's order, so the
+ * same arguments give the reference's cloud bit for bit.  With n_energy = 2,
+ * vz takes its own speed around energy_eV[1]; 1 or 3 energies use one speed.
+ * mass_amu[n] in u; pos, vel [n][3] in m and m/s. */
+int  sbk_init_cloud(int n, unsigned long seed, const double *semiaxis, const double *offset,
+                    int n_energy, const double *energy_eV, const double *mass_amu,
+                    double *pos, double *vel);
+/* The reference's mass tables: the AME file (IonTable, bin/IonTable.cpp:14-91,
+ * names "<A><El>" as "133Cs") and the PDG particle file (PDGTable,
+ * bin/PDGTable.cpp:79-183, masses converted by 1000/931.494 u per GeV).
+ * Either path may be NULL.  Lookups take the first row of the name, AME
+ * before PDG. */
+typedef struct sbk_mass_table sbk_mass_table;
+int  sbk_mass_table_open(const char *ame_path, const char *pdg_path, sbk_mass_table **out);
+int  sbk_mass_table_lookup(const sbk_mass_table *t, const char *name, double *mass_amu);
+void sbk_mass_table_close(sbk_mass_table *t);
+/* A [composition] name (bin/initialization.cpp:333-391): charge from the
+ * suffix "+", "-", "<k>+", "<k>-" (1 without one), mass from the tables
+ * (t may be NULL) or else the name's leading digits ("133Xx+" -> 133 u). */
+int  sbk_parse_species(const char *name, const sbk_mass_table *t, double *mass_amu, int *charge);
+/* Runs a reference simulation file (main + DoSimulation, bin/main.cpp:13-31,
+ * bin/initialization.cpp:208-670): sections [cloud], [composition], [ode]
+ * (Dormand-Prince), [coulomb], [trap] (ideal), [output] and [operation0..31]
+ * (normal, tof, trap_ramp, nop), with the reference's defaults, print
+ * schedule and output files, on this engine at the given precision and
+ * time_mode.  Not provided (an error): [import], other ODE methods and trap
+ * types, dissociation, print_at_x.  t may be NULL (names then need a leading
+ * mass number).  verbose prints each operation on stderr. */
+int  sbk_run_ini(const char *ini_path, const sbk_mass_table *t, int precision, int time_mode,
+                 int verbose);
+/* What sbk_run_ini would run (host only, no GPU): the file's settings after
+ * the reference's defaults and rules, as one JSON object in json[len]: N,
+ * temp, semiaxis, offset, species [{name, count, mass_amu, charge}] in ion
+ * order, timestep, adaptive, coulomb, coulomb_scale, trap {Vrf, Vdc, kappa,
+ * r0, z0, freq_rf, E_kick, newVrf, newVdc}, prefix, print_interval,
+ * separate, print_after_operation, operations [{type, duration,
+ * print_timestep, ode_timestep}], skipped_line (first line that is neither
+ * a section nor a pair, 0 if none). */
+int  sbk_ini_describe(const char *ini_path, const sbk_mass_table *t, char *json, int len);
 
 const char *sbk_last_error(void);
 /* Device name, compute-unit count and clock (MHz) of the current device. */

@@ -10,7 +10,12 @@ Two checkers live here:
   force() (bin/force.cpp:34-175) and Dormand_Prince_5/error_succes/step
   (bin/ode.cpp:61-218,426-494), see ``simbuca_oracle.c``;
 * ``_ref/ref_harness`` -- the reference's own CPU sources compiled in place
-  (bin/*.cpp with -D__CPUNBODY_ON__) behind a small driver, when it was built.
+  (bin/*.cpp with -D__CPUNBODY_ON__) behind a small driver, when it was built;
+* ``_ref/simbuca`` and ``_ref/ref_init`` -- the reference program itself and a
+  driver of its InitCloud and mass tables (``make -C oracle refsim``).  They
+  read the reference's mass tables from ``../libraries`` at start-up, so they
+  run in a scratch directory whose parent links ``libraries`` to the
+  reference's; nothing is written outside that scratch directory.
 """
 from __future__ import annotations
 
@@ -26,6 +31,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsimbuca_oracle.so")
 REF_HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+REF_SIMBUCA = os.path.join(HERE, "_ref", "simbuca")
+REF_INIT = os.path.join(HERE, "_ref", "ref_init")
+REF_LIBRARIES = os.path.join(os.environ.get("SIMBUCA_REFERENCE", "/root/reference"), "libraries")
 
 TIME_FROZEN = 0
 TIME_STAGE = 1
@@ -326,3 +334,74 @@ def ref_time(mode, mass_amu, charge, pos, vel, f: Fields, lifetime, calls, h0=1e
     out = _run_ref(_ref_input(mode, mass_amu, charge, pos, vel, f, lifetime, h0, rtol, atol,
                               True, calls), False, timeout=timeout)
     return json.loads(out.strip().splitlines()[-1])
+
+
+# ---------------------------------------------------------------------------
+# the reference program and its initializer (oracle/_ref/simbuca, ref_init)
+# ---------------------------------------------------------------------------
+
+def refsim_available() -> bool:
+    return (all(os.path.exists(p) and os.access(p, os.X_OK) for p in (REF_SIMBUCA, REF_INIT))
+            and os.path.isdir(REF_LIBRARIES))
+
+
+def _scratch(d):
+    """d/libraries -> the reference's tables; d/run is the working directory."""
+    os.symlink(REF_LIBRARIES, os.path.join(d, "libraries"))
+    run = os.path.join(d, "run")
+    os.mkdir(run)
+    return run
+
+
+def ref_init_cloud(n, seed, energy_eV, semiaxis, offset, mass_amu):
+    """The reference's InitCloud: (pos [n,3], vel [n,3]) exactly as it draws them."""
+    e = [float(x) for x in np.atleast_1d(energy_eV)]
+    args = [REF_INIT, "init", str(n), str(seed), str(len(e))] + ["%.17g" % x for x in e]
+    args += ["%.17g" % float(x) for x in semiaxis] + ["%.17g" % float(x) for x in offset]
+    args += ["%.17g" % float(x) for x in mass_amu]
+    with tempfile.TemporaryDirectory() as d:
+        p = subprocess.run(args, capture_output=True, text=True, timeout=120, cwd=_scratch(d))
+    if p.returncode != 0:
+        raise RuntimeError(f"ref_init failed: {p.stderr}")
+    rows = [ln.split() for ln in p.stdout.splitlines()]
+    a = np.array([[float(x) for x in r] for r in rows if len(r) == 6])
+    if a.shape != (n, 6):
+        raise RuntimeError("ref_init printed %s rows" % (a.shape,))
+    return a[:, :3].copy(), a[:, 3:].copy()
+
+
+def ref_masses(names):
+    """{name: (table, mass in u)} from the reference's mass tables
+    (table "ame", "pdg" or "none" with mass nan)."""
+    with tempfile.TemporaryDirectory() as d:
+        p = subprocess.run([REF_INIT, "mass"] + list(names), capture_output=True, text=True,
+                           timeout=120, cwd=_scratch(d))
+    if p.returncode != 0:
+        raise RuntimeError(f"ref_init failed: {p.stderr}")
+    out = {}
+    for ln in p.stdout.splitlines():
+        r = ln.split()
+        if len(r) >= 2 and r[1] in ("ame", "pdg", "none"):
+            out[r[0]] = (r[1], float(r[2]) if r[1] != "none" else float("nan"))
+    return out
+
+
+def ref_run_ini(ini_text, timeout=600):
+    """Runs the reference program on a simulation file; returns {path relative
+    to the working directory: text} of the .txt files it wrote."""
+    with tempfile.TemporaryDirectory() as d:
+        run = _scratch(d)
+        with open(os.path.join(run, "sim.ini"), "w") as fh:
+            fh.write(ini_text)
+        p = subprocess.run([REF_SIMBUCA, "sim.ini"], capture_output=True, text=True,
+                           timeout=timeout, cwd=run)
+        if p.returncode != 0:
+            raise RuntimeError(f"reference program failed: {p.stdout[-2000:]} {p.stderr}")
+        out = {}
+        for root, _, files in os.walk(run):
+            for name in files:
+                if name.endswith(".txt"):
+                    path = os.path.join(root, name)
+                    with open(path) as fh:
+                        out[os.path.relpath(path, run)] = fh.read()
+        return out

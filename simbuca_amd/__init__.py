@@ -14,6 +14,7 @@ parameters of bin/trapparameters.h:6-27 plus the Penning-trap terms.
 from __future__ import annotations
 
 import ctypes
+import json
 import os
 
 import numpy as np
@@ -93,7 +94,7 @@ class CloudStatus(ctypes.Structure):
         ("error", ctypes.c_int),
         ("steps", ctypes.c_longlong), ("attempts", ctypes.c_longlong),
         ("n", ctypes.c_int), ("slices", ctypes.c_int),
-        ("coulomb_evals", ctypes.c_longlong),
+        ("coulomb_evals", ctypes.c_longlong), ("t_prev", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -119,11 +120,13 @@ EXPORTS = [
     "cunbody1_force", "cunbody1_force_begin", "cunbody1_force_end", "sbk_cunbody_set_precision",
     "initialize", "finalize", "simulation_Async", "simulation_Async_begin", "simulation_Async_end",
     "sbk_default_config", "sbk_default_fields", "sbk_create", "sbk_destroy", "sbk_add_cloud",
-    "sbk_set_fields", "sbk_set_step", "sbk_advance", "sbk_step", "sbk_attempts",
+    "sbk_set_fields", "sbk_set_step", "sbk_advance", "sbk_advance_until", "sbk_step", "sbk_attempts",
     "sbk_synchronize", "sbk_reorder", "sbk_get_state", "sbk_get_status", "sbk_get_stats", "sbk_reset_stats",
     "sbk_eval_force", "sbk_eval_coulomb", "sbk_set_launch", "sbk_time_coulomb", "sbk_autotune",
     "sbk_load_tuning", "sbk_spatial_order", "sbk_writer_open", "sbk_writer_particles",
-    "sbk_writer_cloud", "sbk_writer_close", "sbk_import", "sbk_last_error", "sbk_device_info",
+    "sbk_writer_cloud", "sbk_writer_close", "sbk_import", "sbk_init_cloud", "sbk_mass_table_open",
+    "sbk_mass_table_lookup", "sbk_mass_table_close", "sbk_parse_species", "sbk_run_ini", "sbk_ini_describe",
+    "sbk_last_error", "sbk_device_info",
 ]
 
 
@@ -159,6 +162,7 @@ def lib():
         "sbk_set_fields": (c_int, [vp, c_int, ctypes.POINTER(Fields)]),
         "sbk_set_step": (c_int, [vp, c_int, c_dbl]),
         "sbk_advance": (c_int, [vp, c_dbl, c_ll]),
+        "sbk_advance_until": (c_int, [vp, c_dbl, c_ll]),
         "sbk_step": (c_int, [vp]),
         "sbk_attempts": (c_int, [vp, c_int]),
         "sbk_synchronize": (c_int, [vp]),
@@ -180,6 +184,13 @@ def lib():
         "sbk_writer_cloud": (c_int, [vp, c_dbl, dp, dp]),
         "sbk_writer_close": (c_int, [vp, c_dbl, dp, dp]),
         "sbk_import": (c_int, [ctypes.c_char_p, c_int, ip, ctypes.c_char_p, c_int, dp, dp, dp]),
+        "sbk_init_cloud": (c_int, [c_int, ctypes.c_ulong, dp, dp, c_int, dp, dp, dp, dp]),
+        "sbk_mass_table_open": (c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]),
+        "sbk_mass_table_lookup": (c_int, [vp, ctypes.c_char_p, dp]),
+        "sbk_mass_table_close": (None, [vp]),
+        "sbk_parse_species": (c_int, [ctypes.c_char_p, vp, dp, ip]),
+        "sbk_run_ini": (c_int, [ctypes.c_char_p, vp, c_int, c_int, c_int]),
+        "sbk_ini_describe": (c_int, [ctypes.c_char_p, vp, ctypes.c_char_p, c_int]),
         "sbk_last_error": (ctypes.c_char_p, []),
         "sbk_device_info": (c_int, [ctypes.c_char_p, c_int, ip, ip]),
     }
@@ -299,6 +310,67 @@ def import_snapshot(prefix, capacity=1 << 20, name_len=64):
     return out, pos[:k].copy(), vel[:k].copy(), t.value
 
 
+def init_cloud(n, mass_amu, energy_eV, semiaxis=(1e-3, 1e-3, 1e-3), offset=(0.0, 0.0, 0.0),
+               seed=0):
+    """The reference's InitCloud (host, no GPU): (pos [n,3] m, vel [n,3] m/s).
+    energy_eV is one energy, or (radial, axial) for separate vz speeds."""
+    mass = np.ascontiguousarray(np.broadcast_to(np.asarray(mass_amu, dtype=np.float64), (n,)))
+    e = np.atleast_1d(np.asarray(energy_eV, dtype=np.float64)).copy()
+    semi = np.ascontiguousarray(semiaxis, dtype=np.float64)
+    off = np.ascontiguousarray(offset, dtype=np.float64)
+    if semi.size != 3 or off.size != 3:
+        raise ValueError("semiaxis and offset have three components")
+    pos = np.empty((n, 3)); vel = np.empty((n, 3))
+    _check(lib().sbk_init_cloud(int(n), int(seed), _dp(semi), _dp(off), int(e.size), _dp(e),
+                                _dp(mass), _dp(pos), _dp(vel)))
+    return pos, vel
+
+
+class MassTable:
+    """The reference's AME and PDG mass tables (either path may be None)."""
+
+    def __init__(self, ame_path=None, pdg_path=None):
+        h = ctypes.c_void_p()
+        enc = lambda p: None if p is None else str(p).encode()
+        _check(lib().sbk_mass_table_open(enc(ame_path), enc(pdg_path), ctypes.byref(h)))
+        self._h = h
+
+    def mass(self, name):
+        m = ctypes.c_double()
+        _check(lib().sbk_mass_table_lookup(self._h, str(name).encode(), ctypes.byref(m)))
+        return m.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sbk_mass_table_close(self._h)
+            self._h = None
+
+    __del__ = close
+
+
+def parse_species(name, table=None):
+    """(mass in u, charge) of a [composition] name."""
+    m = ctypes.c_double(); q = ctypes.c_int()
+    _check(lib().sbk_parse_species(str(name).encode(), table._h if table else None,
+                                   ctypes.byref(m), ctypes.byref(q)))
+    return m.value, q.value
+
+
+def describe_ini(path, table=None):
+    """The settings sbk_run_ini would run, as a dict (host only)."""
+    buf = ctypes.create_string_buffer(1 << 20)
+    _check(lib().sbk_ini_describe(str(path).encode(), table._h if table else None, buf, len(buf)))
+    return json.loads(buf.value.decode())
+
+
+def run_ini(path, table=None, precision=FP64_EXACT, time_mode=TIME_FROZEN, verbose=False):
+    """Runs a reference simulation file on the GPU (see sbk_run_ini); the
+    output files go where its [output] prefix says, relative to the
+    working directory."""
+    _check(lib().sbk_run_ini(str(path).encode(), table._h if table else None, int(precision),
+                             int(time_mode), 1 if verbose else 0))
+
+
 def device_info():
     name = ctypes.create_string_buffer(128)
     cus = ctypes.c_int(); clk = ctypes.c_int()
@@ -372,6 +444,9 @@ class Engine:
 
     def advance(self, duration, max_attempts=1 << 62):
         _check(lib().sbk_advance(self._h, float(duration), int(max_attempts)))
+
+    def advance_until(self, t_end, max_attempts=1 << 62):
+        _check(lib().sbk_advance_until(self._h, float(t_end), int(max_attempts)))
 
     def step(self):
         _check(lib().sbk_step(self._h))

@@ -544,6 +544,7 @@ int finalize(sbk_engine *e)
         s.hnext = e->cfg.h0;       // _ode_vars::Init_ode, bin/ode.cpp:539-546
         s.errold = 1.0e-4;
         s.t_end = s.t;
+        s.t_prev = s.t;
         s.budget = -1;
         st[ci] = s;
     }
@@ -1034,13 +1035,11 @@ int sbk_step(sbk_engine *e)
     return fail(SBK_ESTATE, "no accepted step after 100000 attempts");
 }
 
-int sbk_advance(sbk_engine *e, double duration, long long max_attempts)
+static int advance_impl(sbk_engine *e, double value, int set_end, long long max_attempts)
 {
-    int rc = prepare(e);
-    if (rc) return rc;
-    if (!(duration >= 0.0)) return fail(SBK_EINVAL, "duration must be >= 0");
-    if (sbk::launch_begin(e->dev, duration, -1, 1, e->stream))
+    if (sbk::launch_begin(e->dev, value, -1, set_end, e->stream))
         return fail(SBK_EHIP, "begin kernel launch failed");
+    int rc = 0;
     long long launched = 0;
     int batch = 4;
     while (launched < max_attempts) {
@@ -1060,6 +1059,22 @@ int sbk_advance(sbk_engine *e, double duration, long long max_attempts)
         batch = std::min(batch * 2, 64);
     }
     return 0;
+}
+
+int sbk_advance(sbk_engine *e, double duration, long long max_attempts)
+{
+    int rc = prepare(e);
+    if (rc) return rc;
+    if (!(duration >= 0.0)) return fail(SBK_EINVAL, "duration must be >= 0");
+    return advance_impl(e, duration, 1, max_attempts);
+}
+
+int sbk_advance_until(sbk_engine *e, double t_end, long long max_attempts)
+{
+    int rc = prepare(e);
+    if (rc) return rc;
+    if (std::isnan(t_end)) return fail(SBK_EINVAL, "t_end is NaN");
+    return advance_impl(e, t_end, 2, max_attempts);
 }
 
 int sbk_reorder(sbk_engine *e)
@@ -1120,6 +1135,7 @@ int sbk_get_status(sbk_engine *e, int cloud, sbk_cloud_status *st)
     st->attempts = s.attempts;
     st->n = e->clouds[cloud].n;
     st->coulomb_evals = s.evals;
+    st->t_prev = s.t_prev;
     ItemList *L = nullptr;
     st->slices = (get_list(e, e->cfg.precision, true, &L) == 0) ? L->slices[cloud] : 0;
     return 0;

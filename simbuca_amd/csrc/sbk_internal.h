@@ -68,6 +68,7 @@ struct CloudState {
     int budget;               // accepted steps still wanted (-1: unlimited)
     int k1_valid;             // KA[1] holds the derivative at (P0, V0, t)
     long long evals;          // Coulomb evaluations done (6 or 7 per attempt)
+    double t_prev;            // start time of the last accepted step
 };
 
 // One Coulomb work item: targets [i0,i1) against sources [j0,j1) of one
@@ -130,6 +131,7 @@ int coulomb_blocks_per_cu(int precision, int targets_per_thread, int unit_weight
 int launch_stage(int stage, const EngineDev &d, void *stream);
 int launch_controller(const EngineDev &d, void *stream);
 int launch_commit(const EngineDev &d, void *stream);
+// set_end: 0 keep t_end, 1 t_end = t + duration, 2 t_end = duration (absolute)
 int launch_begin(const EngineDev &d, double duration, int budget, int set_end, void *stream);
 int launch_eval_derivs(const EngineDev &d, int with_fields, void *stream);
 // out[g] = in[src[g]] for g < n

@@ -685,6 +685,7 @@ __global__ __launch_bounds__(256) void controller_kernel(EngineDev d)
     // when that is exact (commit_kernel copies it)
     cs.k1_valid = accept ? p.fsal : 1;
     if (accept) {
+        cs.t_prev = cs.t;
         cs.t = cs.t + cs.h;
         cs.h = cs.hnext;
         cs.steps++;
@@ -715,7 +716,8 @@ __global__ void begin_kernel(EngineDev d, double duration, int budget, int set_e
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= d.nclouds) return;
     CloudState &cs = d.state[c];
-    if (set_end) cs.t_end = cs.t + duration;
+    if (set_end == 1) cs.t_end = cs.t + duration;
+    if (set_end == 2) cs.t_end = duration;   // an absolute end time
     cs.budget = budget;
     cs.accepted_last = 0;
     cs.active = (d.params[c].n > 0) && !cs.error && (cs.t < cs.t_end) && budget != 0;

@@ -1,12 +1,20 @@
 """Writes the golden fixtures of tests/golden/ by running the reference's own
-CPU force()/step() (oracle/_ref/ref_harness, compiled from the reference
-sources by `make -C oracle ref`).
+code, compiled from its sources by `make -C oracle ref refsim`:
+
+  force, steps  its CPU force()/step() (oracle/_ref/ref_harness);
+  output        its IonCloud writer (output_n7/);
+  init          its InitCloud (init_cloud.npz) and mass tables (masses.json),
+                through oracle/_ref/ref_init;
+  ini           the reference program itself (oracle/_ref/simbuca) on the
+                simulation files ini/<case>.ini: the text files it writes go
+                to ini/<case>/ (its log file and the _bundled.txt summary,
+                which reads columns its writer does not write, are left out).
 
 Each fixture holds the inputs and the reference's outputs as plain arrays
-(numpy .npz, no pickles); the output-file fixtures (output_n7/) are the text
-files the reference's IonCloud writes.  Run from the repository root:
+(numpy .npz, no pickles), JSON or the reference's own text files.  Run from
+the repository root:
 
-    make -C oracle ref && python tests/golden/make_golden.py [force] [steps] [output]
+    make -C oracle ref refsim && python tests/golden/make_golden.py [force] [steps] [output] [init] [ini]
 """
 from __future__ import annotations
 
@@ -85,7 +93,71 @@ def output_case(name, seed, n, species, prints, lifetime):
                 fh.write(text)
 
 
+INIT_CASES = [
+    # (seed, energies, semiaxis, offset): one per InitCloud branch
+    (0, (0.1,), (1e-3, 1e-3, 1e-3), (0.0, 0.0, 0.0)),
+    (12345, (0.05, 0.2), (4e-4, 2e-3, 7e-4), (1e-4, -2e-4, 3e-4)),
+    (77, (0.01, 0.01, 0.01), (2e-3, 5e-4, 1e-3), (-1e-3, 0.0, 2e-4)),
+]
+MASS_NAMES = ["133Cs", "85Rb", "87Rb", "39K", "1H", "2H", "3H", "3He", "4He", "7Li", "12C", "16O",
+              "23Na", "40Ar", "40Ca", "88Sr", "138Ba", "208Pb", "238U", "1n", "proton", "neutron",
+              "e-", "e+", "mu-", "pi+", "133Xx", "Cs"]
+INI_CASES = ["paul_cycle", "adaptive_separate", "two_ramps"]
+
+
+def init_case(name="init_cloud.npz", n=240):
+    rng = np.random.Generator(np.random.PCG64(21))
+    amu = rng.choice(np.array([133.0, 85.0, 39.0, 7.016, 1.00727]), n)
+    arrays = {"mass_amu": amu}
+    for k, (seed, e, semi, off) in enumerate(INIT_CASES):
+        pos, vel = oracle.ref_init_cloud(n, seed, e, semi, off, amu)
+        arrays.update({f"seed{k}": np.int64(seed), f"energy{k}": np.array(e),
+                       f"semiaxis{k}": np.array(semi), f"offset{k}": np.array(off),
+                       f"pos{k}": pos, f"vel{k}": vel})
+    np.savez(os.path.join(OUT, name), **arrays)
+    print(name, n, len(INIT_CASES))
+
+
+def masses_case(name="masses.json"):
+    import json
+    got = oracle.ref_masses(MASS_NAMES)
+    rows = [{"name": k, "table": got[k][0],
+             "mass_amu": None if got[k][0] == "none" else repr(got[k][1])} for k in MASS_NAMES]
+    with open(os.path.join(OUT, name), "w") as fh:
+        json.dump(rows, fh, indent=1)
+        fh.write("\n")
+    print(name, len(rows))
+
+
+def ini_case(case):
+    import shutil
+    with open(os.path.join(OUT, "ini", case + ".ini")) as fh:
+        text = fh.read()
+    files = oracle.ref_run_ini(text)
+    d = os.path.join(OUT, "ini", case)
+    shutil.rmtree(d, ignore_errors=True)
+    kept = 0
+    for rel, body in sorted(files.items()):
+        if rel.endswith(("_logfile.txt", "_bundled.txt")):
+            continue
+        path = os.path.join(d, rel)
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        with open(path, "w") as fh:
+            fh.write(body)
+        kept += 1
+    print("ini", case, kept, "files")
+
+
 def main(which=("force", "steps", "output")):
+    if "init" in which or "ini" in which:
+        if not oracle.refsim_available():
+            raise SystemExit("oracle/_ref/simbuca or ref_init is not built (make -C oracle refsim)")
+        if "init" in which:
+            init_case()
+            masses_case()
+        if "ini" in which:
+            for case in INI_CASES:
+                ini_case(case)
     if not oracle.ref_available():
         raise SystemExit("oracle/_ref/ref_harness is not built (make -C oracle ref)")
     if "output" in which:

@@ -1,0 +1,121 @@
+"""Simulation files run on the GPU (sbk_run_ini) against the files the
+reference program itself wrote for them (tests/golden/ini/<case>/, made by
+tests/golden/make_golden.py from oracle/_ref/simbuca), and the engine calls
+the runner relies on: sbk_advance_until and the t_prev status field.
+
+Tolerance of the file comparison: the reference prints "%.6e"; FP64-exact
+runs agree with it far below that, so two prints of the same number differ
+by at most one unit in the 7th significant digit.  Each number must agree to
+3e-6 of itself plus 3e-6 of its column's largest magnitude in that file (for
+entries near zero); indices, names, ion counts and line counts must be
+identical, so the print schedule is the reference's.
+"""
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+import simbuca_amd as S
+from test_gpu_engine import _rand_cloud
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+INI = os.path.join(HERE, "golden", "ini")
+CASES = ["paul_cycle", "adaptive_separate", "two_ramps"]
+
+
+def _tables(root):
+    """{relative path: [data rows as word lists]} of the .txt files under root."""
+    out = {}
+    for d, _, files in os.walk(root):
+        for f in files:
+            if f.endswith(".txt"):
+                path = os.path.join(d, f)
+                with open(path) as fh:
+                    out[os.path.relpath(path, root)] = [ln.split() for ln in fh if ln[:1].isdigit()]
+    return out
+
+
+def _compare(mine, ref, rtol=3e-6, what=""):
+    assert len(mine) == len(ref), what
+    # integer and name columns: the first 2 (particle files) or 3 (cloud files)
+    n_words = 3 if "_cloud" in what else 2
+    for a, b in zip(mine, ref):
+        assert len(a) == len(b) and a[:n_words] == b[:n_words], what
+    A = np.array([[float(x) for x in r[n_words:]] for r in mine])
+    B = np.array([[float(x) for x in r[n_words:]] for r in ref])
+    scale = np.abs(B).max(axis=0, keepdims=True)
+    bad = np.abs(A - B) > rtol * np.abs(B) + rtol * scale
+    assert not bad.any(), (what, np.argwhere(bad)[:5], A[bad][:5], B[bad][:5])
+
+
+def _run(case, tmp_path, monkeypatch, precision):
+    print(f"run_ini {case} precision {precision}", flush=True)
+    run = tmp_path / "run"
+    run.mkdir()
+    shutil.copy(os.path.join(INI, case + ".ini"), run / "sim.ini")
+    monkeypatch.chdir(run)
+    S.run_ini("sim.ini", precision=precision, time_mode=S.TIME_FROZEN)
+    return _tables(str(run)), _tables(os.path.join(INI, case))
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_run_ini_matches_reference_program(case, tmp_path, monkeypatch):
+    mine, ref = _run(case, tmp_path, monkeypatch, S.FP64_EXACT)
+    assert sorted(mine) == sorted(ref)
+    for name in ref:
+        _compare(mine[name], ref[name], what=name)
+
+
+def test_run_ini_mixed_precision(tmp_path, monkeypatch):
+    # same schedule and names; the Coulomb sum in mixed precision moves the
+    # numbers by far less than the cloud's size
+    mine, ref = _run("paul_cycle", tmp_path, monkeypatch, S.MIXED)
+    assert sorted(mine) == sorted(ref)
+    for name in ref:
+        _compare(mine[name], ref[name], rtol=1e-4, what=name)
+
+
+def test_run_ini_errors(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    with pytest.raises(S.SimbucaError, match="cannot read"):
+        S.run_ini("missing.ini")
+    (tmp_path / "x.ini").write_text("[cloud]\nNParticles = 4\n[composition]\n9Q+ = 1\n"
+                                    "[ode]\nmethod = RK\n")
+    with pytest.raises(S.SimbucaError, match="Dormand-Prince"):
+        S.run_ini("x.ini")
+
+
+def test_advance_until_and_t_prev(rng):
+    amu, m, ch, pos, vel = _rand_cloud(rng, 64)
+    f = S.default_fields(trap=1, Vrf=0.0, Vdc=5.0, kappa=0.001, coulomb=1)
+    with S.Engine(precision="fp64_exact", adaptive=False, use_graph=False, h0=1e-9) as e:
+        e.add_cloud(pos, vel, m, ch, f)
+        e.advance_until(5.5e-9)                         # stops at the first step past it
+        st = e.status(0)
+        t = 0.0
+        for _ in range(6):
+            t_prev, t = t, t + 1e-9
+        assert st["lifetime"] == t and st["t_prev"] == t_prev and st["steps"] == 6
+        e.advance_until(5.5e-9)                         # already there: no step
+        assert e.status(0)["steps"] == 6
+        e.advance_until(np.nextafter(t, np.inf))        # exactly one step
+        st = e.status(0)
+        assert st["steps"] == 7 and st["t_prev"] == t and st["lifetime"] == t + 1e-9
+        p1, v1 = e.state(0)
+    # the same 7 steps through sbk_step
+    with S.Engine(precision="fp64_exact", adaptive=False, use_graph=False, h0=1e-9) as e:
+        e.add_cloud(pos, vel, m, ch, f)
+        for _ in range(7):
+            e.step()
+        p2, v2 = e.state(0)
+    assert np.array_equal(p1, p2) and np.array_equal(v1, v2)
+    # adaptive: the end lies inside the last step
+    with S.Engine(precision="fp64_exact", adaptive=True, use_graph=True, h0=1e-9) as e:
+        e.add_cloud(pos, vel, m, ch, f)
+        e.advance_until(3.3e-7)
+        st = e.status(0)
+        assert st["t_prev"] < 3.3e-7 <= st["lifetime"]
+        assert st["steps"] > 1
