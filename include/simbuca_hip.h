@@ -307,7 +307,8 @@ int  sbk_writer_close(sbk_writer *w, double lifetime, const double *pos, const d
 /* Reads the last complete print of <prefix>_pAll.txt written as above: n ions
  * in ID order, names[n][name_len], pos/vel [n][3] in m and m/s, lifetime in s.
  * The reference's own importer (bin/parser.cpp:9-261) expects 14 columns its
- * writer never writes; this reads the writer's 11. */
+ * writer never writes; this reads the writer's 11.  With capacity 0 and every
+ * output pointer but n NULL, only *n is set (a size query). */
 int  sbk_import(const char *prefix, int capacity, int *n, char *names, int name_len,
                 double *pos, double *vel, double *lifetime);
 

@@ -297,9 +297,13 @@ class Writer:
             lib().sbk_writer_close(self._h, 0.0, None, None)
 
 
-def import_snapshot(prefix, capacity=1 << 20, name_len=64):
+def import_snapshot(prefix, capacity=None, name_len=64):
     """Last complete print of <prefix>_pAll.txt: (names, pos [m], vel [m/s], lifetime [s])."""
     n = ctypes.c_int()
+    if capacity is None:
+        _check(lib().sbk_import(str(prefix).encode(), 0, ctypes.byref(n), None, name_len, None,
+                                None, None))
+        capacity = n.value
     names = ctypes.create_string_buffer(capacity * name_len)
     pos = np.zeros((capacity, 3)); vel = np.zeros((capacity, 3)); t = ctypes.c_double()
     _check(lib().sbk_import(str(prefix).encode(), capacity, ctypes.byref(n), names, name_len,

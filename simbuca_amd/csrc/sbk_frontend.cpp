@@ -436,22 +436,21 @@ struct Plan {
     double timestep = 1e-9;
     bool adaptive = false;
     sbk_fields f;
+    std::string import_prefix;        // [import]: the cloud is a print read back
+    std::vector<double> pos, vel;     // the imported state
+    double lifetime0 = 0.0;           // lifetime at the start of the run
     std::string prefix;
+    int digits = 6;
     double print_interval = 1e-6;
     bool separate = false, print_after_operation = false;
     std::vector<Operation> ops;
     int skipped_line = 0;   // first line that is neither a section nor a pair
 };
 
-int make_plan(const char *ini_path, const sbk_mass_table *table, Plan &P)
+// [cloud] and [composition] (bin/initialization.cpp:308-396): species in
+// sorted order, n = fraction * N, the last species takes the rest
+int read_cloud(Ini &ini, const sbk_mass_table *table, Plan &P)
 {
-    if (!ini_path) return ffail(SBK_EINVAL, "null path");
-    Ini ini;
-    if (!ini.parse(ini_path)) return ffail(SBK_EIO, "cannot read %s", ini_path);
-    // the reference reads on past such lines (it does not check the parse error)
-    P.skipped_line = ini.error_line;
-
-    // [cloud] (:308-319)
     if (!ini.has("cloud")) return ffail(SBK_EINVAL, "no [cloud] section");
     P.N = (int)ini.real("cloud", "NParticles", 1024);
     P.temp = ini.real("cloud", "temp", 1e-3);
@@ -462,10 +461,6 @@ int make_plan(const char *ini_path, const sbk_mass_table *table, Plan &P)
     P.off[1] = ini.real("cloud", "y0", 0.0);
     P.off[2] = ini.real("cloud", "z0", 0.0);
     if (P.N < 0) return ffail(SBK_EINVAL, "[cloud] NParticles < 0");
-    if (ini.has("import")) return ffail(SBK_EINVAL, "[import] is not provided: load a print with sbk_import");
-
-    // [composition] (:325-396): species in sorted order, n = fraction * N,
-    // the last species takes the rest
     if (!ini.has("composition")) return ffail(SBK_EINVAL, "no [composition] section");
     const std::set<std::string> &comps = ini.fields["composition"];
     for (auto it = comps.begin(); it != comps.end(); ++it) {
@@ -486,6 +481,49 @@ int make_plan(const char *ini_path, const sbk_mass_table *table, Plan &P)
     }
     if ((int)P.names.size() != P.N)
         return ffail(SBK_EINVAL, "composition gives %zu of %d ions", P.names.size(), P.N);
+    return 0;
+}
+
+int make_plan(const char *ini_path, const sbk_mass_table *table, Plan &P)
+{
+    if (!ini_path) return ffail(SBK_EINVAL, "null path");
+    Ini ini;
+    if (!ini.parse(ini_path)) return ffail(SBK_EIO, "cannot read %s", ini_path);
+    // the reference reads on past such lines (it does not check the parse error)
+    P.skipped_line = ini.error_line;
+
+    if (ini.has("import")) {
+        // [import] (:465-484): the cloud is the last print of <prefix>_pAll.txt.
+        // The reference's ImportData (bin/parser.cpp:9-261) reads 14 columns
+        // its writer never writes and looks names up without their charge
+        // suffix; here the writer's 11 columns are read back and each name
+        // goes through the composition rules.  [cloud] and [composition]
+        // are then not used.
+        P.import_prefix = ini.get("import", "prefix", "out");
+        int n = 0;
+        if (int rc = sbk_import(P.import_prefix.c_str(), 0, &n, nullptr, 64, nullptr, nullptr, nullptr))
+            return rc;
+        const int name_len = 256;
+        std::vector<char> buf((size_t)n * name_len);
+        P.pos.resize(3 * (size_t)n);
+        P.vel.resize(3 * (size_t)n);
+        if (int rc = sbk_import(P.import_prefix.c_str(), n, &n, buf.data(), name_len, P.pos.data(),
+                                P.vel.data(), &P.lifetime0))
+            return rc;
+        P.N = n;
+        for (int i = 0; i < n; ++i) {
+            const std::string name(&buf[(size_t)i * name_len]);
+            double m = 0.0;
+            int q = 1;
+            if (int rc = species(name, table, &m, &q)) return rc;
+            P.names.push_back(name);
+            P.mass_amu.push_back(m);
+            P.charge.push_back(q);
+        }
+    } else if (int rc = read_cloud(ini, table, P)) {
+        return rc;
+    }
+
 
     // [ode], [coulomb], [trap], [output] (:398-507)
     if (!ini.has("ode")) return ffail(SBK_EINVAL, "no ODE configuration provided.");
@@ -516,6 +554,9 @@ int make_plan(const char *ini_path, const sbk_mass_table *table, Plan &P)
     P.print_interval = ini.real("output", "timestep", 1e-6);
     P.separate = ini.boolean("output", "separate", false);
     P.print_after_operation = ini.boolean("output", "print_after_operation", false);
+    // the writer's precision, "%.<digits>e" (not a reference key; 17 makes exact restart files)
+    P.digits = (int)ini.real("output", "digits", 6);
+    if (P.digits < 1 || P.digits > 17) return ffail(SBK_EINVAL, "[output] digits must be 1..17");
     if (ini.boolean("output", "print_at_x", false))
         return ffail(SBK_EINVAL, "[output] print_at_x is not provided");
 
@@ -651,8 +692,16 @@ extern "C" int sbk_ini_describe(const char *ini_path, const sbk_mass_table *tabl
         o += std::string("\"") + trap[k].first + "\": ";
         json_num(o, trap[k].second);
     }
-    o += "}, \"prefix\": ";
+    o += "}, \"import\": ";
+    if (P.import_prefix.empty())
+        o += "null";
+    else
+        json_str(o, P.import_prefix);
+    o += ", \"lifetime\": ";
+    json_num(o, P.lifetime0);
+    o += ", \"prefix\": ";
     json_str(o, P.prefix);
+    o += ", \"digits\": " + std::to_string(P.digits);
     o += ", \"print_interval\": ";
     json_num(o, P.print_interval);
     o += std::string(", \"separate\": ") + (P.separate ? "true" : "false");
@@ -688,12 +737,18 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
     const int N = P.N;
     sbk_fields f = P.f;
 
-    // the cloud: seed 0 and three equal energies (:571-593), so InitCloud
-    // takes one speed per ion
+    // the cloud: imported, or drawn with seed 0 and three equal energies
+    // (:571-593), so InitCloud takes one speed per ion
     std::vector<double> pos(3 * (size_t)N), vel(3 * (size_t)N);
-    const double energy[3] = {P.temp, P.temp, P.temp};
-    if (int rc = sbk_init_cloud(N, 0UL, P.semi, P.off, 3, energy, P.mass_amu.data(), pos.data(), vel.data()))
-        return rc;
+    if (!P.import_prefix.empty()) {
+        pos = P.pos;
+        vel = P.vel;
+    } else {
+        const double energy[3] = {P.temp, P.temp, P.temp};
+        if (int rc = sbk_init_cloud(N, 0UL, P.semi, P.off, 3, energy, P.mass_amu.data(), pos.data(),
+                                    vel.data()))
+            return rc;
+    }
     std::vector<double> mass_kg(N);
     for (int i = 0; i < N; ++i) mass_kg[i] = (double)((long double)P.mass_amu[i] * kAmu);
 
@@ -706,13 +761,14 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
     EngineGuard eg;
     if (int rc = sbk_create(&cfg, &eg.e)) return rc;
     sbk_engine *e = eg.e;
-    if (int rc = sbk_add_cloud(e, N, pos.data(), vel.data(), mass_kg.data(), P.charge.data(), &f, 0.0, nullptr))
+    if (int rc = sbk_add_cloud(e, N, pos.data(), vel.data(), mass_kg.data(), P.charge.data(), &f,
+                               P.lifetime0, nullptr))
         return rc;
     std::vector<const char *> cnames(N);
     for (int i = 0; i < N; ++i) cnames[i] = P.names[i].c_str();
     WriterGuard wg;
-    if (int rc = sbk_writer_open(P.prefix.c_str(), N, cnames.data(), P.mass_amu.data(), P.separate ? 1 : 0, 6,
-                                 &wg.w))
+    if (int rc = sbk_writer_open(P.prefix.c_str(), N, cnames.data(), P.mass_amu.data(), P.separate ? 1 : 0,
+                                 P.digits, &wg.w))
         return rc;
 
     auto print = [&](double t) -> int {
@@ -723,9 +779,12 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
 
     // operation::launch and move_particles (bin/operation.cpp:33-83,
     // bin/ion_fly.cpp:17-108): after every step, print once when
-    // lifetime > print_interval * nr_interval, nr_interval counting prints
-    // over the whole run.  The engine stops at the first step past the next
-    // print time (or at the end), which is the step the reference prints at.
+    // lifetime - t0 > print_interval * nr_interval, nr_interval counting
+    // prints over the whole run and t0 the lifetime the run started at (0
+    // unless imported; the reference measures only its first operation from
+    // there and the others from 0).  The engine stops at the first step past
+    // the next print time (or at the end), which is the step the reference
+    // prints at.
     double print_interval = P.print_interval;
     int nr_interval = 0;
     sbk_cloud_status st;
@@ -748,12 +807,20 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
             std::fprintf(stderr, "operation %s for %g s from t = %g s\n", op.name.c_str(), op.time, t_start);
         while (st.lifetime < endtime) {
             const double t_print = print_interval * nr_interval;
-            // one step when the print time is already behind
-            double target = std::nextafter(std::max(t_print, st.lifetime), INFINITY);
+            // the first lifetime with lifetime - t0 > t_print (one step when
+            // that is already behind)
+            double target = st.lifetime;
+            if (!(target - P.lifetime0 > t_print)) {
+                // the least such double: x - t0 rounds monotonically in x
+                target = P.lifetime0 + t_print;
+                while (target - P.lifetime0 > t_print) target = std::nextafter(target, -INFINITY);
+                while (!(target - P.lifetime0 > t_print)) target = std::nextafter(target, INFINITY);
+            }
+            if (target <= st.lifetime) target = std::nextafter(st.lifetime, INFINITY);
             if (target > endtime) target = endtime;
             if (int rc = sbk_advance_until(e, target, 1LL << 62)) return rc;
             if (int rc = sbk_get_status(e, 0, &st)) return rc;
-            if (st.lifetime > t_print) {
+            if (st.lifetime - P.lifetime0 > t_print) {
                 if (int rc = print(st.lifetime)) return rc;
                 nr_interval++;
             }

@@ -302,6 +302,7 @@ int sbk_import(const char *prefix, int capacity, int *n_out, char *names, int na
     const int n = (int)(rows.size() - first);
     *n_out = n;
     if (n == 0) return out_fail(SBK_EIO, "%s holds no ion lines", fn.c_str());
+    if (capacity == 0 && !names && !pos && !vel && !lifetime) return 0;   // a size query
     if (n > capacity) return out_fail(SBK_ENOMEM, "%s holds %d ions, capacity %d", fn.c_str(), n, capacity);
     std::vector<const Row *> last;
     for (size_t i = first; i < rows.size(); ++i) last.push_back(&rows[i]);

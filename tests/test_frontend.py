@@ -376,13 +376,38 @@ def test_ini_operations(tmp_path):
     (("type = normal", "type = whatever"), "No operation"),
     (("[composition]\n133Xx+ = 1\n", ""), "composition"),
     (("[output]\n", "[output]\nprint_at_x = true\n"), "print_at_x"),
-    (("[output]\n", "[import]\nprefix = old\n[output]\n"), "import"),
+    (("[output]\n", "[import]\nprefix = old\n[output]\n"), "cannot open old_pAll"),
+    (("prefix = p", "prefix = p\ndigits = 18"), "digits"),
     (("[ode]\n", "[trap]\ntype = real\n[ode]\n"), "ideal"),
     (("type = normal", "type = normal\node_timestep = 1e-9"), "print_timestep"),
 ])
 def test_ini_not_provided_or_wrong(tmp_path, edit, msg):
     with pytest.raises(S.SimbucaError, match=msg):
         _describe(tmp_path, BASE.replace(*edit))
+
+
+def test_ini_import_reads_a_print(tmp_path, rng):
+    """[import]: the cloud is the last print of <prefix>_pAll.txt (the writer's
+    11 columns), names through the composition rules, lifetime from its t."""
+    n = 5
+    names = ["133Xx+", "133Xx+", "40Zz2+", "7Ww-", "133Xx+"]
+    amu = np.array([133.0, 133.0, 40.0, 7.0, 133.0])
+    pos = rng.normal(0, 1e-3, (n, 3)); vel = rng.normal(0, 100.0, (n, 3))
+    w = S.Writer(str(tmp_path / "snap"), names, amu, digits=17)
+    w.particles(1e-6, pos * 0.5, vel * 0.5)
+    w.close(2.5e-6, pos, vel)
+    text = BASE.replace("[cloud]\nNParticles = 10\n[composition]\n133Xx+ = 1\n",
+                        "[import]\nprefix = %s\n" % (tmp_path / "snap"))
+    d = _describe(tmp_path, text)
+    assert d["import"] == str(tmp_path / "snap") and d["N"] == n
+    assert d["lifetime"] == pytest.approx(2.5e-6, rel=1e-15)
+    assert [(s["name"], s["count"], s["mass_amu"], s["charge"]) for s in d["species"]] == [
+        ("133Xx+", 2, 133.0, 1), ("40Zz2+", 1, 40.0, 2), ("7Ww-", 1, 7.0, -1), ("133Xx+", 1, 133.0, 1)]
+    # a name with no mass is an error, as the reference's importer stops there
+    w = S.Writer(str(tmp_path / "bad"), ["Qq+"], [1.0])
+    w.close(1e-6, np.zeros((1, 3)), np.zeros((1, 3)))
+    with pytest.raises(S.SimbucaError, match="no mass"):
+        _describe(tmp_path, text.replace("snap", "bad"))
 
 
 def test_ini_missing_file(tmp_path):

@@ -12,6 +12,7 @@ identical, so the print schedule is the reference's.
 """
 import os
 import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -76,6 +77,86 @@ def test_run_ini_mixed_precision(tmp_path, monkeypatch):
     assert sorted(mine) == sorted(ref)
     for name in ref:
         _compare(mine[name], ref[name], rtol=1e-4, what=name)
+
+
+SPLIT = """[cloud]
+NParticles = 10
+temp = 0.01
+sx = 0.0005
+sy = 0.0005
+sz = 0.001
+[composition]
+133Xx+ = 0.5
+40Zz2+ = 0.5
+[ode]
+method = DP
+timestep = 1e-8
+[coulomb]
+enable = true
+[trap]
+Vrf = 50
+Vdc = 5
+r0 = 0.004
+z0 = 0.004
+freq_rf = 1e6
+[output]
+prefix = {prefix}
+timestep = 1e-7
+digits = 17
+"""
+OPS = ["[operation0]\ntype = normal\nduration = 4.55e-7\n",
+       "[operation1]\ntype = tof\nduration = 3.35e-7\nEkick = 100\n",
+       "[operation2]\ntype = nop\nduration = 2.05e-7\n"]
+
+
+def _last_print(prefix, n):
+    names, pos, vel, t = S.import_snapshot(prefix)
+    assert len(names) == n
+    return names, pos, vel, t
+
+
+def test_run_ini_import_continues_a_run(tmp_path, monkeypatch):
+    """A run split in two by [import] of a 17-digit print ends where the whole
+    run ends (the print carries positions to within an ulp)."""
+    print("run_ini import split", flush=True)
+    monkeypatch.chdir(tmp_path)
+    (tmp_path / "whole.ini").write_text(SPLIT.format(prefix="whole") + "".join(OPS))
+    (tmp_path / "a.ini").write_text(SPLIT.format(prefix="a") + OPS[0])
+    b = SPLIT.format(prefix="b").replace(SPLIT[:SPLIT.index("[ode]")], "[import]\nprefix = a\n")
+    (tmp_path / "b.ini").write_text(b + "".join(OPS[1:]))
+    for name in ("whole", "a", "b"):
+        S.run_ini(name + ".ini", precision=S.FP64_EXACT)
+    nw, pw, vw, tw = _last_print("whole", 10)
+    na, pa, va, ta = _last_print("a", 10)
+    nb, pb, vb, tb = _last_print("b", 10)
+    assert nw == na == nb and nw[0] == "133Xx+" and nw[-1] == "40Zz2+"
+    assert ta == pytest.approx(4.6e-7, rel=1e-12)
+    assert tb == pytest.approx(tw, rel=1e-14)
+    assert np.allclose(pb, pw, rtol=1e-9, atol=1e-9 * np.abs(pw).max())
+    assert np.allclose(vb, vw, rtol=1e-9, atol=1e-9 * np.abs(vw).max())
+    d = S.describe_ini("b.ini")
+    assert d["import"] == "a" and d["lifetime"] == ta and d["N"] == 10
+
+
+def test_program_runs_a_simulation_file(tmp_path):
+    """simbuca_hip, the command-line program, as `simbuca sim.ini` is run."""
+    exe = os.path.join(os.path.dirname(HERE), "simbuca_amd", "simbuca_hip")
+    assert os.path.exists(exe), "build it with make -C simbuca_amd/csrc"
+    run = tmp_path / "run"
+    run.mkdir()
+    shutil.copy(os.path.join(INI, "two_ramps.ini"), run / "sim.ini")
+    print("simbuca_hip two_ramps", flush=True)
+    p = subprocess.run([exe, "--precision", "fp64_exact", "sim.ini"], cwd=str(run),
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    assert "operation trap_ramp" in p.stderr and "Program ended" in p.stderr
+    mine, ref = _tables(str(run)), _tables(os.path.join(INI, "two_ramps"))
+    assert sorted(mine) == sorted(ref)
+    for name in ref:
+        _compare(mine[name], ref[name], what=name)
+    p = subprocess.run([exe, "--precision", "fp16", "sim.ini"], cwd=str(run),
+                       capture_output=True, text=True, timeout=60)
+    assert p.returncode == 2 and "usage" in p.stderr
 
 
 def test_run_ini_errors(tmp_path, monkeypatch):
