@@ -2,7 +2,7 @@
  * simbuca_hip.h -- C ABI of libsimbuca_hip.so, the MI355X (gfx950) force and
  * Dormand-Prince 5(4) engine for ion-cloud simulation.
  *
- * Two export sets:
+ * Export sets:
  *
  *  1. cunbody1_force{,_begin,_end}: the exact prototypes the reference binds
  *     for its GPU Coulomb backend (bin/cunbody.h:11-33, called from
@@ -17,6 +17,12 @@
  *     stages, steps and retries.  Many independent clouds of different
  *     sizes share one engine; each has its own time, step size and
  *     controller.
+ *
+ *  3. sbk_writer_*, sbk_import: the reference's output files (host only).
+ *
+ *  4. sbk_init_cloud, sbk_mass_table_*, sbk_parse_species, sbk_run_ini,
+ *     sbk_ini_describe: the reference's simulation set-up -- its cloud
+ *     initializer, mass tables and .ini simulation files -- on the engine.
  *
  * All sbk_* functions return 0 on success and a negative SBK_E* code on
  * failure; sbk_last_error() gives the message.  Host arrays are owned by
