@@ -307,6 +307,14 @@ int  sbk_writer_open(const char *prefix, int n, const char *const *names, const 
                      int per_particle_files, int digits, sbk_writer **out);
 int  sbk_writer_particles(sbk_writer *w, double lifetime, const double *pos, const double *vel);
 int  sbk_writer_cloud(sbk_writer *w, double lifetime, const double *pos, const double *vel);
+/* Renames ion k and sets the mass (kg) its energies are printed with, as the
+ * reference's dissociation operation does (bin/operation.cpp:58-70).  The
+ * species files keep the species the writer opened with. */
+int  sbk_writer_set_ion(sbk_writer *w, int k, const char *name, double mass_kg);
+/* IonCloud::PrintX (bin/ioncloud.cpp:120-146): one line "ID name t(ms) x y z(mm)
+ * vx vy vz(m/s)" for ion k in <prefix>_x.txt.  As in the reference, the call
+ * that opens the file writes only its header. */
+int  sbk_writer_x(sbk_writer *w, int k, double lifetime, const double *pos, const double *vel);
 /* IonCloud::Delete: a last print of every ion (if pos/vel are non-null), the
  * closing block, and frees the writer. */
 int  sbk_writer_close(sbk_writer *w, double lifetime, const double *pos, const double *vel);
@@ -344,12 +352,14 @@ void sbk_mass_table_close(sbk_mass_table *t);
 int  sbk_parse_species(const char *name, const sbk_mass_table *t, double *mass_amu, int *charge);
 /* Runs a reference simulation file (main + DoSimulation, bin/main.cpp:13-31,
  * bin/initialization.cpp:208-670): sections [cloud], [composition], [ode]
- * (Dormand-Prince), [coulomb], [trap] (ideal), [output] and [operation0..31]
- * (normal, tof, trap_ramp, nop), with the reference's defaults, print
- * schedule and output files, on this engine at the given precision and
- * time_mode.  Not provided (an error): [import], other ODE methods and trap
- * types, dissociation, print_at_x.  t may be NULL (names then need a leading
- * mass number).  verbose prints each operation on stderr. */
+ * (Dormand-Prince), [coulomb], [trap] (ideal), [output] (print_at_x
+ * included) and [operation0..31] (normal, tof, trap_ramp, nop,
+ * dissociation), with the reference's defaults, print schedule and output
+ * files, on this engine at the given precision and time_mode.  [import]
+ * reads the writer's own prints (the reference's importer cannot); an
+ * [output] key "digits" (default 6) sets the printed precision.  Not provided
+ * (an error): other ODE methods and trap types.  t may be NULL (names then
+ * need a leading mass number).  verbose prints each operation on stderr. */
 int  sbk_run_ini(const char *ini_path, const sbk_mass_table *t, int precision, int time_mode,
                  int verbose);
 /* What sbk_run_ini would run (host only, no GPU): the file's settings after

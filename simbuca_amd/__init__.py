@@ -124,7 +124,7 @@ EXPORTS = [
     "sbk_synchronize", "sbk_reorder", "sbk_get_state", "sbk_get_status", "sbk_get_stats", "sbk_reset_stats",
     "sbk_eval_force", "sbk_eval_coulomb", "sbk_set_launch", "sbk_time_coulomb", "sbk_autotune",
     "sbk_load_tuning", "sbk_spatial_order", "sbk_writer_open", "sbk_writer_particles",
-    "sbk_writer_cloud", "sbk_writer_close", "sbk_import", "sbk_init_cloud", "sbk_mass_table_open",
+    "sbk_writer_cloud", "sbk_writer_set_ion", "sbk_writer_x", "sbk_writer_close", "sbk_import", "sbk_init_cloud", "sbk_mass_table_open",
     "sbk_mass_table_lookup", "sbk_mass_table_close", "sbk_parse_species", "sbk_run_ini", "sbk_ini_describe",
     "sbk_last_error", "sbk_device_info",
 ]
@@ -182,6 +182,8 @@ def lib():
                                     c_int, ctypes.POINTER(vp)]),
         "sbk_writer_particles": (c_int, [vp, c_dbl, dp, dp]),
         "sbk_writer_cloud": (c_int, [vp, c_dbl, dp, dp]),
+        "sbk_writer_set_ion": (c_int, [vp, c_int, ctypes.c_char_p, c_dbl]),
+        "sbk_writer_x": (c_int, [vp, c_int, c_dbl, dp, dp]),
         "sbk_writer_close": (c_int, [vp, c_dbl, dp, dp]),
         "sbk_import": (c_int, [ctypes.c_char_p, c_int, ip, ctypes.c_char_p, c_int, dp, dp, dp]),
         "sbk_init_cloud": (c_int, [c_int, ctypes.c_ulong, dp, dp, c_int, dp, dp, dp, dp]),
@@ -281,6 +283,15 @@ class Writer:
     def cloud(self, lifetime, pos, vel):
         pos, vel = self._state(pos, vel)
         _check(lib().sbk_writer_cloud(self._h, float(lifetime), _dp(pos), _dp(vel)))
+
+    def set_ion(self, k, name, mass_kg):
+        """Renames ion k and sets the mass (kg) of its printed energy."""
+        _check(lib().sbk_writer_set_ion(self._h, int(k), str(name).encode(), float(mass_kg)))
+
+    def x(self, k, lifetime, pos, vel):
+        """The reference's PrintX line for ion k (the first call writes the header only)."""
+        pos, vel = self._state(pos, vel)
+        _check(lib().sbk_writer_x(self._h, int(k), float(lifetime), _dp(pos), _dp(vel)))
 
     def close(self, lifetime=0.0, pos=None, vel=None):
         if not self._h:

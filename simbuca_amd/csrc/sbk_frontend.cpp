@@ -422,8 +422,11 @@ int sbk_parse_species(const char *name, const sbk_mass_table *table, double *mas
 namespace {
 
 struct Operation {
-    std::string name;   // normal, tof, trap_ramp, free
+    std::string name;   // normal, tof, trap_ramp, free, dissociation
     double time = 1e-3, print_time = 0.0, timestep = 0.0;
+    // dissociation (bin/initialization.cpp:529-536, bin/operation.cpp:58-70)
+    double fraction = 1.0, product_mass = 0.0;
+    std::string reactant = "none", product = "none";
 };
 
 // What DoSimulation reads from a simulation file (bin/initialization.cpp:208-546)
@@ -442,7 +445,8 @@ struct Plan {
     std::string prefix;
     int digits = 6;
     double print_interval = 1e-6;
-    bool separate = false, print_after_operation = false;
+    bool separate = false, print_after_operation = false, print_at_x = false;
+    double print_x = 0.0;
     std::vector<Operation> ops;
     int skipped_line = 0;   // first line that is neither a section nor a pair
 };
@@ -557,8 +561,8 @@ int make_plan(const char *ini_path, const sbk_mass_table *table, Plan &P)
     // the writer's precision, "%.<digits>e" (not a reference key; 17 makes exact restart files)
     P.digits = (int)ini.real("output", "digits", 6);
     if (P.digits < 1 || P.digits > 17) return ffail(SBK_EINVAL, "[output] digits must be 1..17");
-    if (ini.boolean("output", "print_at_x", false))
-        return ffail(SBK_EINVAL, "[output] print_at_x is not provided");
+    P.print_at_x = ini.boolean("output", "print_at_x", false);
+    P.print_x = ini.real("output", "print_x", 0.0);
 
     // [operation0..31] (:509-546).  Ekick and the ramp's Vrf/Vdc are one
     // setting of the trap each: the last operation that names one sets it
@@ -581,7 +585,13 @@ int make_plan(const char *ini_path, const sbk_mass_table *table, Plan &P)
             P.f.newVrf = ini.real(s, "Vrf", 0.0);
             P.f.newVdc = ini.real(s, "Vdc", 0.0);
         } else if (type == "dissociation") {
-            return ffail(SBK_EINVAL, "[%s] dissociation is not provided", s.c_str());
+            op.name = "dissociation";
+            op.fraction = ini.real(s, "fraction", 1.0);
+            op.reactant = ini.get(s, "reactant", "none");
+            op.product = ini.get(s, "product", "none");
+            op.product_mass = ini.real(s, "mass", 0.0);   // kg: it replaces Ion::mass
+            if (op.product.empty() || op.product.find_first_of(" \t\n") != std::string::npos)
+                return ffail(SBK_EINVAL, "[%s] product must be one word", s.c_str());
         } else if (type == "nop") {
             op.name = "free";
         } else {
@@ -706,6 +716,8 @@ extern "C" int sbk_ini_describe(const char *ini_path, const sbk_mass_table *tabl
     json_num(o, P.print_interval);
     o += std::string(", \"separate\": ") + (P.separate ? "true" : "false");
     o += std::string(", \"print_after_operation\": ") + (P.print_after_operation ? "true" : "false");
+    o += std::string(", \"print_at_x\": ") + (P.print_at_x ? "true" : "false") + ", \"print_x\": ";
+    json_num(o, P.print_x);
     o += ", \"operations\": [";
     for (size_t k = 0; k < P.ops.size(); ++k) {
         if (k) o += ", ";
@@ -717,6 +729,16 @@ extern "C" int sbk_ini_describe(const char *ini_path, const sbk_mass_table *tabl
         json_num(o, P.ops[k].print_time);
         o += ", \"ode_timestep\": ";
         json_num(o, P.ops[k].timestep);
+        if (P.ops[k].name == "dissociation") {
+            o += ", \"fraction\": ";
+            json_num(o, P.ops[k].fraction);
+            o += ", \"reactant\": ";
+            json_str(o, P.ops[k].reactant);
+            o += ", \"product\": ";
+            json_str(o, P.ops[k].product);
+            o += ", \"mass\": ";
+            json_num(o, P.ops[k].product_mass);
+        }
         o += "}";
     }
     o += "], \"skipped_line\": " + std::to_string(P.skipped_line) + "}";
@@ -751,19 +773,8 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
     }
     std::vector<double> mass_kg(N);
     for (int i = 0; i < N; ++i) mass_kg[i] = (double)((long double)P.mass_amu[i] * kAmu);
+    std::vector<std::string> names = P.names;
 
-    sbk_config cfg;
-    sbk_default_config(&cfg);
-    cfg.precision = precision;
-    cfg.time_mode = time_mode;
-    cfg.adaptive = P.adaptive ? 1 : 0;
-    cfg.h0 = P.timestep;
-    EngineGuard eg;
-    if (int rc = sbk_create(&cfg, &eg.e)) return rc;
-    sbk_engine *e = eg.e;
-    if (int rc = sbk_add_cloud(e, N, pos.data(), vel.data(), mass_kg.data(), P.charge.data(), &f,
-                               P.lifetime0, nullptr))
-        return rc;
     std::vector<const char *> cnames(N);
     for (int i = 0; i < N; ++i) cnames[i] = P.names[i].c_str();
     WriterGuard wg;
@@ -771,8 +782,30 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
                                  P.digits, &wg.w))
         return rc;
 
+    // The engine starts with the first operation that moves the ions: the
+    // reference copies masses into its force vectors there, once
+    // (bin/ion_fly.cpp:26-62), so a dissociation before it changes the
+    // dynamics and one after it only the printed names and energies.
+    EngineGuard eg;
+    sbk_config cfg;
+    sbk_default_config(&cfg);
+    cfg.precision = precision;
+    cfg.time_mode = time_mode;
+    cfg.adaptive = P.adaptive ? 1 : 0;
+    cfg.h0 = P.timestep;
+    double pending_h = 0.0;   // an ode_timestep setting made before the engine exists
+    auto start_engine = [&]() -> int {
+        if (eg.e) return 0;
+        if (int rc = sbk_create(&cfg, &eg.e)) return rc;
+        if (int rc = sbk_add_cloud(eg.e, N, pos.data(), vel.data(), mass_kg.data(), P.charge.data(), &f,
+                                   P.lifetime0, nullptr))
+            return rc;
+        if (pending_h > 0.0)
+            if (int rc = sbk_set_step(eg.e, 0, pending_h)) return rc;
+        return 0;
+    };
+
     auto print = [&](double t) -> int {
-        if (int rc = sbk_get_state(e, 0, pos.data(), vel.data())) return rc;
         if (int rc = sbk_writer_particles(wg.w, t, pos.data(), vel.data())) return rc;
         return sbk_writer_cloud(wg.w, t, pos.data(), vel.data());
     };
@@ -784,16 +817,40 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
     // unless imported; the reference measures only its first operation from
     // there and the others from 0).  The engine stops at the first step past
     // the next print time (or at the end), which is the step the reference
-    // prints at.
+    // prints at.  With print_at_x it steps one step at a time and reports the
+    // ions whose x crossed print_x in that step.
     double print_interval = P.print_interval;
     int nr_interval = 0;
     sbk_cloud_status st;
-    if (int rc = sbk_get_status(e, 0, &st)) return rc;
+    std::memset(&st, 0, sizeof(st));
+    st.lifetime = P.lifetime0;
+    std::vector<double> old_x(N);
     for (const Operation &op : P.ops) {
         if (op.print_time) print_interval = op.print_time;
         if (op.timestep) {   // the reference sets h to the print step here (bin/operation.cpp:37-40)
-            if (int rc = sbk_set_step(e, 0, op.print_time)) return rc;
+            if (eg.e) {
+                if (int rc = sbk_set_step(eg.e, 0, op.print_time)) return rc;
+            } else {
+                pending_h = op.print_time;
+            }
         }
+        if (op.name == "dissociation") {
+            // MTRand seeded with 0 at each dissociation, one draw per reactant ion
+            std::mt19937 mt(0u);
+            for (int i = 0; i < N; ++i) {
+                if (names[i] != op.reactant) continue;
+                if (static_cast<double>(mt()) * (1. / 4294967296.) < op.fraction) {
+                    names[i] = op.product;
+                    if (!eg.e) mass_kg[i] = op.product_mass;
+                    if (int rc = sbk_writer_set_ion(wg.w, i, op.product.c_str(), op.product_mass)) return rc;
+                }
+            }
+            if (verbose) std::fprintf(stderr, "operation dissociation of %s into %s\n", op.reactant.c_str(),
+                                      op.product.c_str());
+            continue;
+        }
+        if (int rc = start_engine()) return rc;
+        sbk_engine *e = eg.e;
         sbk_fields g = f;
         g.trap = (op.name == "normal" || op.name == "trap_ramp") ? 1 : 0;
         g.tof = (op.name == "tof") ? 1 : 0;
@@ -807,23 +864,33 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
             std::fprintf(stderr, "operation %s for %g s from t = %g s\n", op.name.c_str(), op.time, t_start);
         while (st.lifetime < endtime) {
             const double t_print = print_interval * nr_interval;
-            // the first lifetime with lifetime - t0 > t_print (one step when
-            // that is already behind)
             double target = st.lifetime;
-            if (!(target - P.lifetime0 > t_print)) {
-                // the least such double: x - t0 rounds monotonically in x
+            if (!P.print_at_x && !(target - P.lifetime0 > t_print)) {
+                // the first lifetime with lifetime - t0 > t_print: the least
+                // such double, as x - t0 rounds monotonically in x
                 target = P.lifetime0 + t_print;
                 while (target - P.lifetime0 > t_print) target = std::nextafter(target, -INFINITY);
                 while (!(target - P.lifetime0 > t_print)) target = std::nextafter(target, INFINITY);
             }
-            if (target <= st.lifetime) target = std::nextafter(st.lifetime, INFINITY);
+            if (target <= st.lifetime) target = std::nextafter(st.lifetime, INFINITY);   // one step
             if (target > endtime) target = endtime;
+            if (P.print_at_x)
+                for (int i = 0; i < N; ++i) old_x[i] = pos[3 * (size_t)i];
             if (int rc = sbk_advance_until(e, target, 1LL << 62)) return rc;
             if (int rc = sbk_get_status(e, 0, &st)) return rc;
-            if (st.lifetime - P.lifetime0 > t_print) {
+            const bool due = st.lifetime - P.lifetime0 > t_print;
+            if (due || P.print_at_x)
+                if (int rc = sbk_get_state(e, 0, pos.data(), vel.data())) return rc;
+            if (due) {
                 if (int rc = print(st.lifetime)) return rc;
                 nr_interval++;
             }
+            if (P.print_at_x)
+                for (int i = 0; i < N; ++i) {
+                    const double x = pos[3 * (size_t)i];
+                    if ((old_x[i] < P.print_x && x > P.print_x) || (old_x[i] > P.print_x && x < P.print_x))
+                        if (int rc = sbk_writer_x(wg.w, i, st.lifetime, pos.data(), vel.data())) return rc;
+                }
         }
         if (g.ramp && st.steps > 0 && st.t_prev >= t_start) {
             // the trap keeps the values of the last step's start (bin/ion_fly.cpp:95-99)
@@ -831,10 +898,13 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
             f.Vrf = g.Vrf * (1 - ramp) + g.newVrf * ramp;
             f.Vdc = g.Vdc * (1 - ramp) + g.newVdc * ramp;
         }
-        if (P.print_after_operation)
+        if (P.print_after_operation) {
+            if (int rc = sbk_get_state(e, 0, pos.data(), vel.data())) return rc;
             if (int rc = print(st.lifetime)) return rc;
+        }
     }
-    if (int rc = sbk_get_state(e, 0, pos.data(), vel.data())) return rc;
+    if (eg.e)
+        if (int rc = sbk_get_state(eg.e, 0, pos.data(), vel.data())) return rc;
     sbk_writer *w = wg.w;
     wg.w = nullptr;
     return sbk_writer_close(w, st.lifetime, pos.data(), vel.data());

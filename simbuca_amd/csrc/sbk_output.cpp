@@ -94,6 +94,7 @@ struct sbk_writer {
     File all;
     std::vector<File> parts;
     std::vector<File> clouds;
+    File xfile;                          // <prefix>_x.txt, opened by the first crossing
     time_t start = 0;
 };
 
@@ -234,6 +235,46 @@ int sbk_writer_cloud(sbk_writer *w, double lifetime, const double *pos, const do
         put(f, w->digits, energy);
         std::fputc('\n', f);
     }
+    return 0;
+}
+
+int sbk_writer_set_ion(sbk_writer *w, int k, const char *name, double mass_kg)
+{
+    if (!w || k < 0 || k >= w->n || !name) return out_fail(SBK_EINVAL, "bad arguments");
+    if (!name[0] || std::strpbrk(name, " \t\n"))
+        return out_fail(SBK_EINVAL, "ion %d: a name must be one non-empty word", k);
+    // the species files keep the species of the opening (IonCloud::ion_types
+    // is filled by AddParticle only), so a renamed ion leaves its old sums
+    w->names[k] = name;
+    w->mass[k] = (long double)mass_kg;
+    return 0;
+}
+
+int sbk_writer_x(sbk_writer *w, int k, double lifetime, const double *pos, const double *vel)
+{
+    if (int rc = check_writer(w, pos, vel)) return rc;
+    if (k < 0 || k >= w->n) return out_fail(SBK_EINVAL, "bad ion index");
+    if (!w->xfile) {
+        // IonCloud::PrintX (bin/ioncloud.cpp:120-146): the call that opens
+        // the file writes its header only
+        const std::string fn = w->prefix + "_x.txt";
+        w->xfile.reset(std::fopen(fn.c_str(), "w"));
+        if (!w->xfile) return out_fail(SBK_EIO, "cannot open %s: %s", fn.c_str(), std::strerror(errno));
+        std::fprintf(w->xfile.get(), "#Print at X position");
+        write_header(w->xfile.get(), "#index \t type \t t (ms) \t x y z (mm) vx vy vz (m/s)");
+        return 0;
+    }
+    const double *p = pos + 3 * (size_t)k, *v = vel + 3 * (size_t)k;
+    FILE *f = w->xfile.get();
+    std::fprintf(f, "%d %s", k, w->names[k].c_str());
+    put(f, w->digits, lifetime * 1000.0);
+    put(f, w->digits, p[0] * 1000.0);
+    put(f, w->digits, p[1] * 1000.0);
+    put(f, w->digits, p[2] * 1000.0);
+    put(f, w->digits, v[0]);
+    put(f, w->digits, v[1]);
+    put(f, w->digits, v[2]);
+    std::fputc('\n', f);
     return 0;
 }
 

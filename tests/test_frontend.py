@@ -372,10 +372,9 @@ def test_ini_operations(tmp_path):
     (("[ode]\nmethod = DP\n", ""), "ODE configuration"),
     (("method = DP", "method = RK"), "only Dormand-Prince"),
     (("[output]\nprefix = p\n", ""), "Output file"),
-    (("type = normal", "type = dissociation"), "dissociation"),
     (("type = normal", "type = whatever"), "No operation"),
     (("[composition]\n133Xx+ = 1\n", ""), "composition"),
-    (("[output]\n", "[output]\nprint_at_x = true\n"), "print_at_x"),
+    (("type = normal", "type = dissociation\nproduct = two words"), "one word"),
     (("[output]\n", "[import]\nprefix = old\n[output]\n"), "cannot open old_pAll"),
     (("prefix = p", "prefix = p\ndigits = 18"), "digits"),
     (("[ode]\n", "[trap]\ntype = real\n[ode]\n"), "ideal"),
@@ -408,6 +407,48 @@ def test_ini_import_reads_a_print(tmp_path, rng):
     w.close(1e-6, np.zeros((1, 3)), np.zeros((1, 3)))
     with pytest.raises(S.SimbucaError, match="no mass"):
         _describe(tmp_path, text.replace("snap", "bad"))
+
+
+def test_ini_dissociation_and_print_at_x(tmp_path):
+    ops = ("[operation0]\ntype = dissociation\nreactant = 133Xx+\nproduct = 85Pd+\n"
+           "fraction = 0.4\nmass = 1.4e-25\n"
+           "[operation1]\ntype = normal\n"
+           "[operation2]\ntype = dissociation\n")
+    text = BASE.replace("[operation0]\ntype = normal\n", ops).replace(
+        "[output]\n", "[output]\nprint_at_x = yes\nprint_x = 2e-4\n")
+    d = _describe(tmp_path, text)
+    assert d["print_at_x"] is True and d["print_x"] == 2e-4
+    assert [o["type"] for o in d["operations"]] == ["dissociation", "normal", "dissociation"]
+    o = d["operations"][0]
+    assert (o["fraction"], o["reactant"], o["product"], o["mass"]) == (0.4, "133Xx+", "85Pd+", 1.4e-25)
+    # the reader's defaults (bin/initialization.cpp:530-534)
+    o = d["operations"][2]
+    assert (o["fraction"], o["reactant"], o["product"], o["mass"]) == (1.0, "none", "none", 0.0)
+    d = _describe(tmp_path, BASE)
+    assert d["print_at_x"] is False and d["print_x"] == 0.0
+
+
+def test_writer_rename_and_x_prints(tmp_path):
+    w = S.Writer(str(tmp_path / "w"), ["133Xx+", "85Yy+"], [133.0, 85.0])
+    pos = np.array([[1e-3, 2e-3, 3e-3], [-1e-3, 0.0, 1e-3]]); vel = np.array([[10.0, 0, 0], [0, 5.0, 0]])
+    w.x(0, 1e-6, pos, vel)                        # opens the file: header only
+    w.x(1, 2e-6, pos, vel)
+    w.set_ion(0, "40Pd+", 40 * S.AMU)
+    w.particles(3e-6, pos, vel)
+    w.cloud(3e-6, pos, vel)
+    w.close()
+    xl = [ln.split() for ln in open(tmp_path / "w_x.txt") if ln[:1].isdigit()]
+    assert xl == [["1", "85Yy+", "2.000000e-03", "-1.000000e+00", "0.000000e+00", "1.000000e+00",
+                   "0.000000e+00", "5.000000e+00", "0.000000e+00"]]
+    assert open(tmp_path / "w_x.txt").readline().startswith("#Print at X position#")
+    pl = [ln.split() for ln in open(tmp_path / "w_pAll.txt") if ln[:1].isdigit()]
+    assert pl[0][1] == "40Pd+" and float(pl[0][-1]) == pytest.approx(0.5 * 40 * S.AMU * 100 / S.EL_CHARGE,
+                                                                      rel=1e-5)
+    # species files keep the opening species: the renamed ion leaves the 133Xx+ sums
+    c0 = [ln.split() for ln in open(tmp_path / "w_cloud0.txt") if ln[:1].isdigit()]
+    assert c0[0][1] == "133Xx+" and float(c0[0][4]) == 0.0 and float(c0[0][-1]) == 0.0
+    with pytest.raises(S.SimbucaError, match="one non-empty word"):
+        S.Writer(str(tmp_path / "v"), ["a"], [1.0]).set_ion(0, "two words", 1.0)
 
 
 def test_ini_missing_file(tmp_path):

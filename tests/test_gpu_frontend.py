@@ -1,7 +1,11 @@
 """Simulation files run on the GPU (sbk_run_ini) against the files the
 reference program itself wrote for them (tests/golden/ini/<case>/, made by
 tests/golden/make_golden.py from oracle/_ref/simbuca), and the engine calls
-the runner relies on: sbk_advance_until and the t_prev status field.
+the runner relies on: sbk_advance_until and the t_prev status field.  The
+cases cover every operation type (normal, trap_ramp, tof, nop, dissociation
+before and after the first move), fixed and adaptive steps, Coulomb on and
+off, both file layouts, print_after_operation and the print_at_x detector
+plane.
 
 Tolerance of the file comparison: the reference prints "%.6e"; FP64-exact
 runs agree with it far below that, so two prints of the same number differ
@@ -24,7 +28,7 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 INI = os.path.join(HERE, "golden", "ini")
-CASES = ["paul_cycle", "adaptive_separate", "two_ramps"]
+CASES = ["paul_cycle", "adaptive_separate", "two_ramps", "tof_detector"]
 
 
 def _tables(root):
@@ -41,7 +45,7 @@ def _tables(root):
 
 def _compare(mine, ref, rtol=3e-6, what=""):
     assert len(mine) == len(ref), what
-    # integer and name columns: the first 2 (particle files) or 3 (cloud files)
+    # integer and name columns: the first 2 (particle and X-plane files) or 3 (cloud files)
     n_words = 3 if "_cloud" in what else 2
     for a, b in zip(mine, ref):
         assert len(a) == len(b) and a[:n_words] == b[:n_words], what
