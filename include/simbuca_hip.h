@@ -358,7 +358,7 @@ int  sbk_parse_species(const char *name, const sbk_mass_table *t, double *mass_a
  * files, on this engine at the given precision and time_mode.  [import]
  * reads the writer's own prints (the reference's importer cannot); an
  * [output] key "digits" (default 6) sets the printed precision.  Not provided
- * (an error): other ODE methods and trap types.  t may be NULL (names then
+ * (an error): ODE methods other than Dormand-Prince.  t may be NULL (names then
  * need a leading mass number).  verbose prints each operation on stderr. */
 int  sbk_run_ini(const char *ini_path, const sbk_mass_table *t, int precision, int time_mode,
                  int verbose);

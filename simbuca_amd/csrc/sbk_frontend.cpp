@@ -543,9 +543,9 @@ int make_plan(const char *ini_path, const sbk_mass_table *table, Plan &P)
         P.f.coulomb = ini.boolean("coulomb", "enable", false) ? 1 : 0;
         P.f.coulomb_scale = ini.real("coulomb", "scalefactor", 1);
     }
-    if (ini.has("trap")) {
-        if (ini.get("trap", "type", "ideal") != "ideal")
-            return ffail(SBK_EINVAL, "[trap] type: only the ideal trap is provided");
+    // another [trap] type leaves the trap's defaults, as in the reference
+    // (only "ideal" reads the section; trap_type stays 0)
+    if (ini.has("trap") && ini.get("trap", "type", "ideal") == "ideal") {
         P.f.Vrf = ini.real("trap", "Vrf", 50);
         P.f.Vdc = ini.real("trap", "Vdc", 5);
         P.f.kappa = ini.real("trap", "kappa", 0.025);

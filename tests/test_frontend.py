@@ -344,6 +344,14 @@ def test_ini_long_lines_and_sections(tmp_path):
         _describe(tmp_path, BASE.replace("[cloud]", "[Cloud]"))
 
 
+def test_ini_trap_types(tmp_path):
+    # only "ideal" reads [trap]; any other type keeps the trap's defaults
+    d = _describe(tmp_path, BASE + "[trap]\ntype = real\nVrf = 99\n")
+    assert d["trap"]["Vrf"] == S.default_fields().Vrf and d["trap"]["kappa"] == S.default_fields().kappa
+    d = _describe(tmp_path, BASE + "[trap]\nVrf = 99\n")
+    assert d["trap"]["Vrf"] == 99.0
+
+
 def test_ini_composition_rules(tmp_path):
     comp = "[composition]\n133Xx+ = 0.25\n85Yy+ = 0.25\n39Zz2+ = 0.9\n"
     d = _describe(tmp_path, BASE.replace("[composition]\n133Xx+ = 1\n", comp))
@@ -377,7 +385,6 @@ def test_ini_operations(tmp_path):
     (("type = normal", "type = dissociation\nproduct = two words"), "one word"),
     (("[output]\n", "[import]\nprefix = old\n[output]\n"), "cannot open old_pAll"),
     (("prefix = p", "prefix = p\ndigits = 18"), "digits"),
-    (("[ode]\n", "[trap]\ntype = real\n[ode]\n"), "ideal"),
     (("type = normal", "type = normal\node_timestep = 1e-9"), "print_timestep"),
 ])
 def test_ini_not_provided_or_wrong(tmp_path, edit, msg):
