@@ -456,7 +456,10 @@ struct Plan {
 int read_cloud(Ini &ini, const sbk_mass_table *table, Plan &P)
 {
     if (!ini.has("cloud")) return ffail(SBK_EINVAL, "no [cloud] section");
-    P.N = (int)ini.real("cloud", "NParticles", 1024);
+    const double nreal = ini.real("cloud", "NParticles", 1024);
+    if (!(nreal >= 0.0 && nreal < 1073741824.0))
+        return ffail(SBK_EINVAL, "[cloud] NParticles must be in [0, 2^30)");
+    P.N = (int)nreal;
     P.temp = ini.real("cloud", "temp", 1e-3);
     P.semi[0] = ini.real("cloud", "sx", 0.001);
     P.semi[1] = ini.real("cloud", "sy", 0.001);
@@ -464,7 +467,6 @@ int read_cloud(Ini &ini, const sbk_mass_table *table, Plan &P)
     P.off[0] = ini.real("cloud", "x0", 0.0);
     P.off[1] = ini.real("cloud", "y0", 0.0);
     P.off[2] = ini.real("cloud", "z0", 0.0);
-    if (P.N < 0) return ffail(SBK_EINVAL, "[cloud] NParticles < 0");
     if (!ini.has("composition")) return ffail(SBK_EINVAL, "no [composition] section");
     const std::set<std::string> &comps = ini.fields["composition"];
     for (auto it = comps.begin(); it != comps.end(); ++it) {
@@ -473,10 +475,13 @@ int read_cloud(Ini &ini, const sbk_mass_table *table, Plan &P)
         double m = 0.0;
         int q = 1;
         if (int rc = species(*it, table, &m, &q)) return rc;
+        if (!(fraction == fraction)) return ffail(SBK_EINVAL, "[composition] %s: not a number", it->c_str());
         int n = (int)(fraction * P.N);
         const int left = P.N - (int)P.names.size();
         if (std::next(it) == comps.end()) n = left;
-        if (n > left) n = left;
+        // the reference compares n with an unsigned count, so a negative n
+        // (a negative fraction) takes every ion left
+        if (n < 0 || n > left) n = left;
         for (int i = 0; i < n; ++i) {
             P.names.push_back(*it);
             P.mass_amu.push_back(m);

@@ -360,6 +360,12 @@ def test_ini_composition_rules(tmp_path):
         [("133Xx+", 2), ("39Zz2+", 8)]
     with pytest.raises(S.SimbucaError, match="fraction"):
         _describe(tmp_path, BASE.replace("133Xx+ = 1", "133Xx+ = 1.5"))
+    # a negative fraction takes every ion left (an unsigned comparison in the reference)
+    comp = "[composition]\n133Xx+ = -0.1\n85Yy+ = 0.5\n"
+    d = _describe(tmp_path, BASE.replace("[composition]\n133Xx+ = 1\n", comp))
+    assert [(s["name"], s["count"]) for s in d["species"]] == [("133Xx+", 10)]
+    with pytest.raises(S.SimbucaError, match="NParticles"):
+        _describe(tmp_path, BASE.replace("NParticles = 10", "NParticles = 1e12"))
 
 
 def test_ini_operations(tmp_path):
