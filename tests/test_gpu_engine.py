@@ -236,6 +236,36 @@ def test_coulomb_precisions_at_65536(exact_65536, precision, local, tol):
     assert np.linalg.norm(f.sum(0)) <= 10 * tol * np.abs(f).sum(0).max()
 
 
+@pytest.mark.parametrize("index", [2, 4, 5])
+def test_coulomb_at_full_workload_sizes(index):
+    """The other benchmark set-ups at full size: 10,000 ions of two species,
+    the 256-cloud ensemble, 262,144 ions of three species (Kahan).  The
+    workload's precision, with its automatic launch shape, against the FP64
+    sum, which is pinned to the oracle on sampled rows of the largest cloud;
+    Newton's third law holds in every cloud."""
+    w = synth.workload(index)
+    print(f"full-size Coulomb, workload {index}: {len(w.clouds)} clouds, "
+          f"{sum(c.n for c in w.clouds)} ions", flush=True)
+    with S.Engine(precision=w.precision) as e:
+        for c in w.clouds:
+            e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
+        mixed = [e.eval_coulomb(k, w.precision) for k in range(len(w.clouds))]
+        fp64 = [e.eval_coulomb(k, "fp64") for k in range(len(w.clouds))]
+    big = int(np.argmax([c.n for c in w.clouds]))
+    c = w.clouds[big]
+    rows = np.random.Generator(np.random.PCG64(index)).choice(c.n, 64, replace=False).astype(np.int32)
+    ref = O.coulomb_rows(c.pos, c.mass, c.charge, c.fields.coulomb_scale, rows)
+    assert _rows_error(fp64[big][rows], ref) < 1e-13
+    errs = [_rows_error(a, b) for a, b in zip(mixed, fp64)]
+    assert max(errs) <= 1e-6, max(errs)
+    assert _rows_error(np.concatenate(mixed), np.concatenate(fp64)) <= 1e-6
+    for cl, a in zip(w.clouds, mixed):
+        if cl.n < 2:
+            continue
+        ma = a * cl.mass[:, None]
+        assert np.linalg.norm(ma.sum(0)) <= 1e-5 * np.abs(ma).sum(0).max()
+
+
 def test_local_coords_keep_the_accuracy(exact_65536):
     # far sub-tiles from FP32 offsets: same normwise error as FP64-formed
     # separations everywhere (measured 5.6e-8 both), well inside 2x
