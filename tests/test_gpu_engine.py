@@ -609,6 +609,28 @@ def test_mixed_short_horizon_deviation(rng):
     assert dev < 1e-9 * np.abs(out["fp64"][0]).max()
 
 
+@pytest.mark.parametrize("index,steps", [(3, 20), (5, 4)])
+def test_full_size_short_horizon(index, steps):
+    """The 65,536-ion and 262,144-ion set-ups at full size: fixed steps of the
+    workload's mixed precision stay within 1e-11 of the cloud's extent from
+    the FP64 engine (stage time, as the benchmark)."""
+    w = synth.workload(index)
+    c = w.clouds[0]
+    print(f"full-size short horizon, workload {index}: {c.n} ions, {steps} steps", flush=True)
+    out = {}
+    for p in ("fp64", w.precision):
+        with S.Engine(precision=p, adaptive=False, h0=1e-9, time_mode=S.TIME_STAGE) as e:
+            e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
+            e.attempts(steps)
+            e.synchronize()
+            out[p] = e.state(0)
+            assert e.status(0)["steps"] == steps
+    scale = np.abs(out["fp64"][0]).max()
+    assert np.abs(out[w.precision][0] - out["fp64"][0]).max() <= 1e-11 * scale
+    vscale = np.abs(out["fp64"][1]).max()
+    assert np.abs(out[w.precision][1] - out["fp64"][1]).max() <= 1e-8 * vscale
+
+
 def test_advance_reaches_t_end_and_counts(rng):
     amu, m, ch, pos, vel = _rand_cloud(rng, 128, species=(133.0,), charges=(1,))
     sf, _ = _both_fields(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0)
