@@ -465,6 +465,32 @@ def test_graph_and_eager_agree(rng):
     assert res[0][2]["attempts"] == res[1][2]["attempts"]
 
 
+def test_graph_replays_match_eager_each_step(rng):
+    """Every replay of the captured attempt equals the eager launch sequence,
+    bit for bit, with eager work on the same stream between replays (a
+    Coulomb evaluation, and a device re-sort every 3 attempts)."""
+    amu, m, ch, pos, vel = _rand_cloud(rng, 700, sigma=1e-3)
+    sf, _ = _both_fields(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0, Ed=10.0, fd=940.0)
+    kw = dict(precision="mixed", rtol=1e-9, atol=1e-9, time_mode=S.TIME_STAGE, reorder_interval=3)
+    with S.Engine(use_graph=True, **kw) as g, S.Engine(use_graph=False, **kw) as e:
+        g.add_cloud(pos, vel, m, ch, sf)
+        e.add_cloud(pos, vel, m, ch, sf)
+        for k in range(12):
+            g.step()
+            e.step()
+            fg = g.eval_coulomb(0, "mixed")
+            fe = e.eval_coulomb(0, "mixed")
+            assert np.array_equal(fg, fe), k
+            (pg, vg), (pe, ve) = g.state(0), e.state(0)
+            assert np.array_equal(pg, pe) and np.array_equal(vg, ve), k
+            sg, se = g.status(0), e.status(0)
+            assert (sg["lifetime"], sg["h"], sg["attempts"]) == (se["lifetime"], se["h"], se["attempts"]), k
+            if k % 4 == 3:
+                g.reorder()
+                e.reorder()
+        assert g.stats()["reorders"] > 0
+
+
 def test_penning_physics_vs_oracle(rng):
     # stage-time mode with B, Penning quadrupole, dipole + quadrupole drive, friction
     amu, m, ch, pos, vel = _rand_cloud(rng, 200, sigma=5e-4, species=(133.0,), charges=(1,))
