@@ -3,8 +3,10 @@
 // on the CPU (bin/main.cpp:13-31).
 //
 //   simbuca_hip [--precision fp64_exact|fp64|mixed|mixed_kahan]
-//               [--time frozen|stage] [--ame FILE] [--pdg FILE] [--quiet] FILE.ini
+//               [--time frozen|stage] [--ame FILE] [--pdg FILE] [--quiet]
+//               [--describe] FILE.ini
 //
+// --describe prints the settings the file resolves to (JSON) and runs nothing.
 // The mass tables default to the reference's own paths relative to the
 // working directory, ../libraries/ame/mass.mas12 and
 // ../libraries/pdg/pdg_table.txt, and are skipped when absent (species names
@@ -12,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include <sys/stat.h>
 
@@ -29,7 +32,7 @@ int usage()
 {
     std::fprintf(stderr,
                  "usage: simbuca_hip [--precision fp64_exact|fp64|mixed|mixed_kahan] [--time frozen|stage]\n"
-                 "                   [--ame FILE] [--pdg FILE] [--quiet] FILE.ini\n");
+                 "                   [--ame FILE] [--pdg FILE] [--quiet] [--describe] FILE.ini\n");
     return 2;
 }
 
@@ -39,7 +42,7 @@ int main(int argc, char **argv)
 {
     int precision = SBK_FP64, time_mode = SBK_TIME_FROZEN, verbose = 1;
     std::string ame = "../libraries/ame/mass.mas12", pdg = "../libraries/pdg/pdg_table.txt", ini;
-    bool ame_given = false, pdg_given = false;
+    bool ame_given = false, pdg_given = false, describe = false;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         auto value = [&](std::string &out) {
@@ -68,6 +71,8 @@ int main(int argc, char **argv)
             pdg_given = true;
         } else if (a == "--quiet") {
             verbose = 0;
+        } else if (a == "--describe") {
+            describe = true;
         } else if (!a.empty() && a[0] == '-') {
             return usage();
         } else if (ini.empty()) {
@@ -83,6 +88,17 @@ int main(int argc, char **argv)
     if (sbk_mass_table_open(ame_path, pdg_path, &t) != 0) {
         std::fprintf(stderr, "simbuca_hip: %s\n", sbk_last_error());
         return 1;
+    }
+    if (describe) {
+        std::vector<char> json(1 << 22);
+        const int rc = sbk_ini_describe(ini.c_str(), t, json.data(), (int)json.size());
+        sbk_mass_table_close(t);
+        if (rc != 0) {
+            std::fprintf(stderr, "simbuca_hip: %s\n", sbk_last_error());
+            return 1;
+        }
+        std::printf("%s\n", json.data());
+        return 0;
     }
     const int rc = sbk_run_ini(ini.c_str(), t, precision, time_mode, verbose);
     sbk_mass_table_close(t);

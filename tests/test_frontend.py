@@ -19,6 +19,7 @@ bin/inireader.cpp) are checked case by case.
 """
 import json
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -462,6 +463,32 @@ def test_writer_rename_and_x_prints(tmp_path):
     assert c0[0][1] == "133Xx+" and float(c0[0][4]) == 0.0 and float(c0[0][-1]) == 0.0
     with pytest.raises(S.SimbucaError, match="one non-empty word"):
         S.Writer(str(tmp_path / "v"), ["a"], [1.0]).set_ion(0, "two words", 1.0)
+
+
+EXE = os.path.join(os.path.dirname(HERE), "simbuca_amd", "simbuca_hip")
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="build it with make -C simbuca_amd/csrc")
+def test_program_describe(tmp_path):
+    """simbuca_hip --describe (no GPU): the settings as JSON; the mass tables
+    come from ../libraries as the reference's, when present."""
+    p = subprocess.run([EXE, "--describe", os.path.join(INI, "two_ramps.ini")], capture_output=True,
+                       text=True, timeout=60)
+    assert p.returncode == 0, p.stderr
+    assert json.loads(p.stdout) == S.describe_ini(os.path.join(INI, "two_ramps.ini"))
+    p = subprocess.run([EXE, "--describe", str(tmp_path / "none.ini")], capture_output=True,
+                       text=True, timeout=60)
+    assert p.returncode == 1 and "cannot read" in p.stderr
+    if os.path.exists(AME) and os.path.exists(PDG):
+        run = tmp_path / "run"
+        run.mkdir()
+        os.symlink(O.REF_LIBRARIES, tmp_path / "libraries")
+        (run / "s.ini").write_text(BASE.replace("133Xx+", "133Cs+"))
+        p = subprocess.run([EXE, "--describe", "s.ini"], cwd=str(run), capture_output=True,
+                           text=True, timeout=60)
+        assert p.returncode == 0, p.stderr
+        t = S.MassTable(AME, PDG)
+        assert json.loads(p.stdout)["species"][0]["mass_amu"] == t.mass("133Cs")
 
 
 def test_ini_missing_file(tmp_path):
