@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <fstream>
 #include <map>
 #include <memory>
@@ -647,6 +648,38 @@ struct WriterGuard {
     }
 };
 
+// <prefix>_logfile.txt: the lines the reference's loggers write for a run
+// (LogFile, bin/logfile.cpp, numbers as an ostream prints a double: %g)
+struct RunLog {
+    FILE *f = nullptr;
+    time_t start = 0;
+    ~RunLog()
+    {
+        if (f) std::fclose(f);
+    }
+    int open(const std::string &prefix)
+    {
+        const std::string fn = prefix + "_logfile.txt";
+        f = std::fopen(fn.c_str(), "w");
+        if (!f) return ffail(SBK_EIO, "cannot open %s: %s", fn.c_str(), std::strerror(errno));
+        start = time(nullptr);
+        struct tm tmv;
+        char buf[64];
+        localtime_r(&start, &tmv);
+        std::fprintf(f, "logfile created on: \t%s", asctime_r(&tmv, buf) ? buf : "?\n");
+        return 0;
+    }
+    void line(const char *fmt, ...)
+    {
+        if (!f) return;
+        va_list ap;
+        va_start(ap, fmt);
+        std::vfprintf(f, fmt, ap);
+        va_end(ap);
+        std::fflush(f);
+    }
+};
+
 int make_folder(const std::string &prefix)
 {
     // mkpath of the prefix's folder (bin/initialization.cpp:562-570)
@@ -763,6 +796,22 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
     if (int rc = make_folder(P.prefix)) return rc;
     const int N = P.N;
     sbk_fields f = P.f;
+    // the set-up lines of InitIonFly, Init_ode, SetCoulomb, UseScaledCoulomb
+    // (bin/ion_fly.cpp:125-132,149-168, bin/ode.cpp:571-585)
+    RunLog log;
+    if (int rc = log.open(P.prefix)) return rc;
+    log.line("Ideal Trap: Vrf = %g, Vdc = %g\n", f.Vrf, f.Vdc);
+    log.line("Choose Dormand Prince 5th order. With %s stepsize and abs error: %g, rel error: %g\n",
+             P.adaptive ? "adaptive" : "fixed", 1e-7, 1e-7);
+    log.line("%s Coulomb Interaction\n", f.coulomb ? "with" : "without");
+    log.line("with scaled coulomb Interaction: factor %g.\n", f.coulomb_scale);
+    if (!P.import_prefix.empty()) {   // ImportData's lines (bin/parser.cpp:228-238)
+        log.line("Data Imported from %s_pAll.txt\n%d particles : \n", P.import_prefix.c_str(), N);
+    } else {   // InitCloud's lines (bin/initialization.cpp:202-205)
+        log.line("%d particles created. Max-Boltz distribution around %g eV. \n", N, P.temp);
+        log.line("spheroid cloud with semi-axis's = {%g;%g;%g} and offset = {%g;%g;%g}  \n", P.semi[0],
+                 P.semi[1], P.semi[2], P.off[0], P.off[1], P.off[2]);
+    }
 
     // the cloud: imported, or drawn with seed 0 and three equal energies
     // (:571-593), so InitCloud takes one speed per ion
@@ -865,6 +914,7 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
         g.ramp_t0 = t_start;
         g.ramp_t1 = t_start + op.time;
         if (int rc = sbk_set_fields(e, 0, &g)) return rc;
+        log.line("Normal trap operation for %g sec,\n", op.time);   // normal_operation, bin/ion_fly.cpp:139-143
         if (verbose)
             std::fprintf(stderr, "operation %s for %g s from t = %g s\n", op.name.c_str(), op.time, t_start);
         while (st.lifetime < endtime) {
@@ -912,5 +962,7 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
         if (int rc = sbk_get_state(eg.e, 0, pos.data(), vel.data())) return rc;
     sbk_writer *w = wg.w;
     wg.w = nullptr;
+    log.line("Close everything...\n");   // ExitIonFly, then Delete (bin/ioncloud.cpp:69)
+    log.line("#simulation ended after %g s\n", (double)(time(nullptr) - log.start));
     return sbk_writer_close(w, st.lifetime, pos.data(), vel.data());
 }

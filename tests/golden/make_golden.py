@@ -7,8 +7,8 @@ code, compiled from its sources by `make -C oracle ref refsim`:
                 through oracle/_ref/ref_init;
   ini           the reference program itself (oracle/_ref/simbuca) on the
                 simulation files ini/<case>.ini: the text files it writes go
-                to ini/<case>/ (its log file and the _bundled.txt summary,
-                which reads columns its writer does not write, are left out).
+                to ini/<case>/ (its _bundled.txt summary, which reads columns
+                its writer does not write, is left out).
 
 Each fixture holds the inputs and the reference's outputs as plain arrays
 (numpy .npz, no pickles), JSON or the reference's own text files.  Run from
@@ -138,7 +138,7 @@ def ini_case(case):
     shutil.rmtree(d, ignore_errors=True)
     kept = 0
     for rel, body in sorted(files.items()):
-        if rel.endswith(("_logfile.txt", "_bundled.txt")):
+        if rel.endswith("_bundled.txt"):
             continue
         path = os.path.join(d, rel)
         os.makedirs(os.path.dirname(path), exist_ok=True)

@@ -32,14 +32,30 @@ CASES = ["paul_cycle", "adaptive_separate", "two_ramps", "tof_detector"]
 
 
 def _tables(root):
-    """{relative path: [data rows as word lists]} of the .txt files under root."""
+    """{relative path: [data rows as word lists]} of the .txt output files
+    under root (the log file is compared by _logs)."""
     out = {}
     for d, _, files in os.walk(root):
         for f in files:
-            if f.endswith(".txt"):
+            if f.endswith(".txt") and not f.endswith("_logfile.txt"):
                 path = os.path.join(d, f)
                 with open(path) as fh:
                     out[os.path.relpath(path, root)] = [ln.split() for ln in fh if ln[:1].isdigit()]
+    return out
+
+
+def _logs(root):
+    """{relative path: lines} of the log files under root, without the lines
+    that carry wall-clock time."""
+    out = {}
+    for d, _, files in os.walk(root):
+        for f in files:
+            if f.endswith("_logfile.txt"):
+                path = os.path.join(d, f)
+                with open(path) as fh:
+                    out[os.path.relpath(path, root)] = [
+                        ln for ln in fh.read().splitlines()
+                        if not ln.startswith(("logfile created on", "#simulation ended after"))]
     return out
 
 
@@ -72,6 +88,9 @@ def test_run_ini_matches_reference_program(case, tmp_path, monkeypatch):
     assert sorted(mine) == sorted(ref)
     for name in ref:
         _compare(mine[name], ref[name], what=name)
+    # the log file, line for line (wall-clock lines aside)
+    ref_logs = _logs(os.path.join(INI, case))
+    assert ref_logs and _logs(str(tmp_path / "run")) == ref_logs
 
 
 def test_run_ini_mixed_precision(tmp_path, monkeypatch):
@@ -158,6 +177,7 @@ def test_program_runs_a_simulation_file(tmp_path):
     assert sorted(mine) == sorted(ref)
     for name in ref:
         _compare(mine[name], ref[name], what=name)
+    assert _logs(str(run)) == _logs(os.path.join(INI, "two_ramps"))
     p = subprocess.run([exe, "--precision", "fp16", "sim.ini"], cwd=str(run),
                        capture_output=True, text=True, timeout=60)
     assert p.returncode == 2 and "usage" in p.stderr
