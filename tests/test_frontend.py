@@ -223,7 +223,7 @@ def _golden_names(case, desc):
     d = os.path.join(INI, case)
     files = []
     for root, _, fs in os.walk(d):
-        files += [os.path.join(root, f) for f in fs]
+        files += [os.path.join(root, f) for f in fs if not f.endswith("_logfile.txt")]
     names = {}
     for path in files:
         if "_cloud" in os.path.basename(path):
