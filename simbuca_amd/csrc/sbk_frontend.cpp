@@ -1,18 +1,20 @@
 // sbk_frontend.cpp -- the reference's simulation set-up on top of the engine:
-// its INI simulation file, its cloud initializer and mass table, and its
-// operation sequence (export set "front end" of include/simbuca_hip.h).
+// its INI simulation file, its cloud initializer and mass tables, and its
+// operation sequence (export set 4 of include/simbuca_hip.h).
 //
 //   sbk_init_cloud     InitCloud (bin/initialization.cpp:117-206): the draws of
 //                      the reference's Mersenne Twister (bin/mtrand.cpp,
 //                      MT19937 seeded by init_genrand, u = x / 2^32) in the
 //                      reference's order, with its arithmetic;
-//   sbk_mass_table_*   the AME table reader (bin/IonTable.cpp:14-91): fixed
-//                      columns A, Z, element, integer and micro-u mass;
-//   sbk_run_ini        DoSimulation + operations::launch + move_particles
-//                      (bin/initialization.cpp:208-650, bin/operation.cpp:33-127,
+//   sbk_mass_table_*   the AME table reader (bin/IonTable.cpp:14-91, fixed
+//                      columns) and the PDG particle table (bin/PDGTable.cpp);
+//   sbk_parse_species  the [composition] name rules;
+//   sbk_ini_describe,  DoSimulation + operations::launch + move_particles
+//   sbk_run_ini        (bin/initialization.cpp:208-650, bin/operation.cpp:33-127,
 //                      bin/ion_fly.cpp:17-122): the same sections and keys,
-//                      the same composition rules, the same print schedule, with
-//                      the engine doing the steps and sbk_writer the files.
+//                      the same composition rules, the same print schedule and
+//                      log lines, with the engine doing the steps and
+//                      sbk_writer the files.
 // Host code; only sbk_run_ini touches the GPU (through the engine).
 #include <algorithm>
 #include <cctype>
@@ -39,7 +41,7 @@
 
 namespace {
 
-// bin/globals.h:4,10
+// bin/globals.h:4,6,10
 const double kPi = 3.14159265358979323;
 const double kEvToJoule = 1.60217646e-19;
 const double kAmu = 1.6605402E-27;
