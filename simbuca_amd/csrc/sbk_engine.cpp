@@ -356,9 +356,12 @@ int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
             }
         }
         S = std::max(1, std::min(S, c.n));
-        // slice length: a multiple of the FP32 sub-tile
+        // slice length: a multiple of the sources one LDS tile gives all Q
+        // wave groups (each group takes its own kSubTile of every tile), so
+        // that only a cloud's last slice ends in a partial tile
+        const int quantum = sbk::kSubTile * q;
         int chunk = (c.n + S - 1) / S;
-        chunk = ((chunk + sbk::kSubTile - 1) / sbk::kSubTile) * sbk::kSubTile;
+        chunk = ((chunk + quantum - 1) / quantum) * quantum;
         S = (c.n + chunk - 1) / chunk;
         L.slices[ci] = S;
         L.max_slices = std::max(L.max_slices, S);
