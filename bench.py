@@ -469,10 +469,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair math (separations from f64 positions, or f32 offsets for far sub-tiles), "
-                 "f64 accumulation and DP5",
+        "dtype": "f32+f64",
         "data": "synthetic (seeded Gaussian / Maxwell-Boltzmann cloud)",
-        "config": {"workload": w.name, "n_ions": n, "precision": "mixed", "rtol": w.rtol,
+        "config": {"workload": w.name, "n_ions": n, "precision": "mixed",
+                   "arithmetic": "f32 pair math (separations from f64 positions, or f32 offsets for "
+                                 "far sub-tiles), f64 accumulation across sub-tiles, f64 DP5",
+                   "rtol": w.rtol,
                    "atol": w.atol, "time_mode": "stage", "step": "one DP5 attempt",
                    "launch_shape": tune, "fp64_launch_shape": tune64, "device": info["name"],
                    "spatial_order": e_cfg["spatial_order"], "reorder_interval": e_cfg["reorder_interval"],
