@@ -102,7 +102,7 @@ INIT_CASES = [
 MASS_NAMES = ["133Cs", "85Rb", "87Rb", "39K", "1H", "2H", "3H", "3He", "4He", "7Li", "12C", "16O",
               "23Na", "40Ar", "40Ca", "88Sr", "138Ba", "208Pb", "238U", "1n", "proton", "neutron",
               "e-", "e+", "mu-", "pi+", "133Xx", "Cs"]
-INI_CASES = ["paul_cycle", "adaptive_separate", "two_ramps", "tof_detector"]
+INI_CASES = ["paul_cycle", "adaptive_separate", "two_ramps", "tof_detector", "adaptive_ramp"]
 
 
 def init_case(name="init_cloud.npz", n=240):

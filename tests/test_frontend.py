@@ -243,13 +243,106 @@ def _golden_names(case, desc):
     return [names[k] for k in sorted(names)], [species[k] for k in sorted(species)]
 
 
-@pytest.mark.parametrize("case", ["paul_cycle", "adaptive_separate", "two_ramps"])
+@pytest.mark.parametrize("case", ["paul_cycle", "adaptive_separate", "two_ramps", "adaptive_ramp"])
 def test_composition_matches_reference_program(case):
     desc = S.describe_ini(os.path.join(INI, case + ".ini"))
     names, species = _golden_names(case, desc)
     assert _expand(desc) == names
     assert [s["name"] for s in desc["species"]] == [s[0] for s in species]
     assert all(n == desc["N"] for _, n in species)
+
+
+def _oracle_run(case, ramp_end_at_lifetime=False):
+    """The runner's operation loop restated over the oracle's own step()
+    (bin/ode.cpp:459-494): prints after the step that passes each print time,
+    after every operation, and at the end; a ramp leaves Vrf/Vdc at the values
+    of its last step's start.  With ramp_end_at_lifetime the ramp ends at the
+    run's lifetime instead: a planted mistake that the golden files must
+    reject.  Returns [(lifetime, pos, vel)] per print."""
+    d = S.describe_ini(os.path.join(INI, case + ".ini"))
+    amu = np.array([sp["mass_amu"] for sp in d["species"] for _ in range(sp["count"])])
+    ch = np.array([sp["charge"] for sp in d["species"] for _ in range(sp["count"])], dtype=np.int32)
+    pos, vel = S.init_cloud(d["N"], amu, (d["temp"],) * 3, d["semiaxis"], d["offset"], seed=0)
+    tr = d["trap"]
+    base = dict(kappa=tr["kappa"], r0=tr["r0"], z0=tr["z0"], freq_rf=tr["freq_rf"],
+                E_kick=tr["E_kick"], newVrf=tr["newVrf"], newVdc=tr["newVdc"],
+                coulomb=1 if d["coulomb"] else 0, coulomb_scale=d["coulomb_scale"])
+    c = O.Cloud(pos, vel, O.mass_kg(amu), ch, O.fields(Vrf=tr["Vrf"], Vdc=tr["Vdc"], **base),
+                h0=d["timestep"], adaptive=d["adaptive"], time_mode=O.TIME_FROZEN)
+    vrf, vdc, interval, nr, t = tr["Vrf"], tr["Vdc"], d["print_interval"], 0, 0.0
+    prints = []
+    for op in d["operations"]:
+        assert op["ode_timestep"] == 0 and op["type"] in ("normal", "trap_ramp", "tof", "free")
+        if op["print_timestep"]:
+            interval = op["print_timestep"]
+        t0, end = t, t + op["duration"]
+        ramp = op["type"] == "trap_ramp"
+        kind = dict(trap=1 if op["type"] in ("normal", "trap_ramp") else 0,
+                    tof=1 if op["type"] == "tof" else 0)
+        c.set_fields(O.fields(Vrf=vrf, Vdc=vdc, **kind, **base))
+        step_vrf, step_vdc = vrf, vdc
+        while t < end:
+            if ramp:   # the trap's values at this step's start (bin/ion_fly.cpp:95-99)
+                x = (t - t0) / (end - t0)
+                step_vrf = vrf * (1 - x) + tr["newVrf"] * x
+                step_vdc = vdc * (1 - x) + tr["newVdc"] * x
+                c.set_fields(O.fields(Vrf=step_vrf, Vdc=step_vdc, **kind, **base))
+            c.step()
+            p_, v_, sc = c.state()
+            t = sc["lifetime"]
+            if t > interval * nr:
+                prints.append((t, p_, v_))
+                nr += 1
+        if ramp:
+            if ramp_end_at_lifetime:
+                x = (t - t0) / (end - t0)
+                step_vrf = vrf * (1 - x) + tr["newVrf"] * x
+                step_vdc = vdc * (1 - x) + tr["newVdc"] * x
+            vrf, vdc = step_vrf, step_vdc
+        if d["print_after_operation"]:
+            p_, v_, _ = c.state()
+            prints.append((t, p_, v_))
+    p_, v_, _ = c.state()
+    prints.append((t, p_, v_))
+    return prints
+
+
+def _golden_particle_rows(case):
+    """[(t, pos, vel)] per print from the reference's files (ms and mm back to s and m)."""
+    rows = {}
+    for root, _, fs in os.walk(os.path.join(INI, case)):
+        for f in fs:
+            if "_cloud" in f or f.endswith("_logfile.txt"):
+                continue
+            for ln in open(os.path.join(root, f)):
+                if ln[:1].isdigit():
+                    w = ln.split()
+                    rows.setdefault(int(w[0]), []).append([float(x) for x in w[2:9]])
+    a = np.array([rows[k] for k in sorted(rows)])          # [ion, print, 7]
+    return [(a[0, j, 0] * 1e-3, a[:, j, 1:4] * 1e-3, a[:, j, 4:7]) for j in range(a.shape[1])]
+
+
+def _prints_agree(mine, ref, rtol=3e-6):
+    if len(mine) != len(ref):
+        return False
+    for (tm, pm, vm), (tr_, pr, vr) in zip(mine, ref):
+        if abs(tm - tr_) > rtol * tr_:
+            return False
+        for a, b in ((pm, pr), (vm, vr)):
+            if np.any(np.abs(a - b) > rtol * np.abs(b) + rtol * np.abs(b).max()):
+                return False
+    return True
+
+
+@pytest.mark.parametrize("case", ["paul_cycle", "two_ramps", "adaptive_ramp"])
+def test_operation_rules_over_the_oracle(case):
+    """The print schedule and the ramp rule, restated over the oracle, give
+    the reference program's own files; ending the ramp at the lifetime
+    instead of the last step's start does not (where a ramp runs)."""
+    ref = _golden_particle_rows(case)
+    assert _prints_agree(_oracle_run(case), ref)
+    if case != "paul_cycle":
+        assert not _prints_agree(_oracle_run(case, ramp_end_at_lifetime=True), ref)
 
 
 def test_describe_golden_files():

@@ -28,7 +28,7 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 INI = os.path.join(HERE, "golden", "ini")
-CASES = ["paul_cycle", "adaptive_separate", "two_ramps", "tof_detector"]
+CASES = ["paul_cycle", "adaptive_separate", "two_ramps", "tof_detector", "adaptive_ramp"]
 
 
 def _tables(root):
