@@ -95,15 +95,32 @@ def make_engine(S, w, precision, timing, use_graph=True, tune=None):
     return e
 
 
+_HIP = None
+
+
+def device_sync():
+    """hipDeviceSynchronize: every stream of the device has drained (the
+    device-wide barrier around a timed region; one process, so no rank
+    barrier is needed)."""
+    global _HIP
+    if _HIP is None:
+        import ctypes
+        _HIP = ctypes.CDLL("libamdhip64.so.7")   # the runtime libsimbuca_hip.so links
+    if _HIP.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
+
+
 def run_engine(S, w, precision, steps, warmup, timing, use_graph=True, tune=None):
     """`warmup` then `steps` timed attempts of every cloud (inputs resident)."""
     e = make_engine(S, w, precision, timing, use_graph, tune)
     e.attempts(warmup)                  # uploads, activates with no end time, warms up
     e.synchronize()
     e.reset_stats()
+    device_sync()
     t0 = time.perf_counter()
     e.attempts(steps, open_ended=False)
     e.synchronize()
+    device_sync()
     dt = time.perf_counter() - t0
     st = e.stats()
     status = e.status(0)
@@ -117,6 +134,7 @@ def run_to_end(S, w, precision, tune=None, pieces=20):
     e.attempts(0)
     e.synchronize()
     e.reset_stats()
+    device_sync()
     t0 = time.perf_counter()
     t_end = e.status(0)["lifetime"] + w.t_sim
     for k in range(1, pieces + 1):
@@ -127,6 +145,7 @@ def run_to_end(S, w, precision, tune=None, pieces=20):
         st = e.stats()
         log(f"{precision} to end: {k}/{pieces}, t = {e.status(0)['lifetime']:.4e} s, "
             f"{st['steps']} steps, {time.perf_counter() - t0:.1f} s")
+    device_sync()
     dt = time.perf_counter() - t0
     st = e.stats()
     e.close()
