@@ -371,6 +371,14 @@ int  sbk_run_ini(const char *ini_path, const sbk_mass_table *t, int precision, i
  * print_timestep, ode_timestep}], skipped_line (first line that is neither
  * a section nor a pair, 0 if none). */
 int  sbk_ini_describe(const char *ini_path, const sbk_mass_table *t, char *json, int len);
+/* The reference's INIReader on a file (bin/ini.c, bin/inireader.cpp):
+ * sbk_ini_value is Get(section, name): the raw value (repeated names and
+ * continuation lines joined by '\n'; names and sections match without regard
+ * to case), or SBK_EINVAL when it is not set.  sbk_ini_names lists, one per
+ * line, the sections (section NULL, GetSections) or the field names of a
+ * section (GetFields), in sorted order with their case kept. */
+int  sbk_ini_value(const char *ini_path, const char *section, const char *name, char *out, int len);
+int  sbk_ini_names(const char *ini_path, const char *section, char *out, int len);
 
 const char *sbk_last_error(void);
 /* Device name, compute-unit count and clock (MHz) of the current device. */

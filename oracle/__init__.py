@@ -405,3 +405,29 @@ def ref_run_ini(ini_text, timeout=600):
                     with open(path) as fh:
                         out[os.path.relpath(path, run)] = fh.read()
         return out
+
+
+def ref_ini(path, queries):
+    """The reference's INIReader on a file: (sections, [(fields of section,
+    Get(section, name) or None)] for each (section, name) query)."""
+    args = [REF_INIT, "ini", os.path.abspath(path)]
+    for sec, name in queries:
+        args += [sec, name]
+    with tempfile.TemporaryDirectory() as d:
+        p = subprocess.run(args, capture_output=True, timeout=120, cwd=_scratch(d))
+    if p.returncode != 0:
+        raise RuntimeError(f"ref_init failed: {p.stderr!r}")
+
+    def unhex(h):
+        return "" if h == "=" else bytes.fromhex(h).decode("latin-1")
+    sections, out, fields = [], [], []
+    for ln in p.stdout.decode().splitlines():
+        kind, _, val = ln.partition(" ")
+        if kind == "S":
+            sections.append(unhex(val))
+        elif kind == "F":
+            fields.append(unhex(val))
+        elif kind == "V":
+            out.append((fields, None if val == "-" else unhex(val)))
+            fields = []
+    return sections, out

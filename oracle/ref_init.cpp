@@ -12,15 +12,22 @@
 //   ref_init mass <name...>
 //       -> one line per name: "<name> ame <mass>", "<name> pdg <mass in u>"
 //          or "<name> none"
+//   ref_init ini <file> [<section> <name>]...
+//       -> the reference's INIReader (bin/inireader.cpp) on the file:
+//          "S <hex>" per section (GetSections), then per queried pair
+//          "F <hex>" per field of the section (GetFields) and "V <hex>" for
+//          Get(section, name) or "V -" when it is not set; strings hex-encoded
 #include "ioncloud.h"
 #include "ion.h"
 #include "IonTable.h"
 #include "PDGTable.h"
 #include "trapparameters.h"
+#include "inireader.h"
 
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -31,10 +38,22 @@ extern IonTable Table;
 extern PDGTable pdgTable;
 extern std::vector<double> x_, y_, z_, vx_, vy_, vz_;
 
+static std::string to_hex(const std::string &v)
+{
+    std::string o;
+    char b[3];
+    for (size_t i = 0; i < v.size(); ++i) {
+        snprintf(b, sizeof(b), "%02x", (unsigned char)v[i]);
+        o += b;
+    }
+    return o.empty() ? std::string("=") : o;
+}
+
 static void usage()
 {
     fprintf(stderr, "usage: ref_init init n seed n_energy e... sx sy sz x0 y0 z0 mass...\n"
-                    "       ref_init mass name...\n");
+                    "       ref_init mass name...\n"
+                    "       ref_init ini file [section name]...\n");
     exit(2);
 }
 
@@ -51,6 +70,21 @@ int main(int argc, char **argv)
                        pdgTable.GetPDGMass(pdgTable.GetPDGId(name)) * 1000.0 / 931.494);
             else
                 printf("%s none\n", argv[i]);
+        }
+        return 0;
+    }
+    if (!strcmp(argv[1], "ini") && argc >= 3 && argc % 2 == 1) {
+        INIReader reader(argv[2]);
+        std::set<std::string> secs = reader.GetSections();
+        for (std::set<std::string>::iterator it = secs.begin(); it != secs.end(); ++it)
+            printf("S %s\n", to_hex(*it).c_str());
+        const std::string missing("\x01");
+        for (int i = 3; i + 1 < argc; i += 2) {
+            std::set<std::string> f = reader.GetFields(argv[i]);
+            for (std::set<std::string>::iterator it = f.begin(); it != f.end(); ++it)
+                printf("F %s\n", to_hex(*it).c_str());
+            const std::string v = reader.Get(argv[i], argv[i + 1], missing);
+            printf("V %s\n", v == missing ? "-" : to_hex(v).c_str());
         }
         return 0;
     }

@@ -125,7 +125,7 @@ EXPORTS = [
     "sbk_eval_force", "sbk_eval_coulomb", "sbk_set_launch", "sbk_time_coulomb", "sbk_autotune",
     "sbk_load_tuning", "sbk_spatial_order", "sbk_writer_open", "sbk_writer_particles",
     "sbk_writer_cloud", "sbk_writer_set_ion", "sbk_writer_x", "sbk_writer_close", "sbk_import", "sbk_init_cloud", "sbk_mass_table_open",
-    "sbk_mass_table_lookup", "sbk_mass_table_close", "sbk_parse_species", "sbk_run_ini", "sbk_ini_describe",
+    "sbk_mass_table_lookup", "sbk_mass_table_close", "sbk_parse_species", "sbk_run_ini", "sbk_ini_describe", "sbk_ini_value", "sbk_ini_names",
     "sbk_last_error", "sbk_device_info",
 ]
 
@@ -193,6 +193,8 @@ def lib():
         "sbk_parse_species": (c_int, [ctypes.c_char_p, vp, dp, ip]),
         "sbk_run_ini": (c_int, [ctypes.c_char_p, vp, c_int, c_int, c_int]),
         "sbk_ini_describe": (c_int, [ctypes.c_char_p, vp, ctypes.c_char_p, c_int]),
+        "sbk_ini_value": (c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, c_int]),
+        "sbk_ini_names": (c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, c_int]),
         "sbk_last_error": (ctypes.c_char_p, []),
         "sbk_device_info": (c_int, [ctypes.c_char_p, c_int, ip, ip]),
     }
@@ -376,6 +378,25 @@ def describe_ini(path, table=None):
     buf = ctypes.create_string_buffer(1 << 20)
     _check(lib().sbk_ini_describe(str(path).encode(), table._h if table else None, buf, len(buf)))
     return json.loads(buf.value.decode())
+
+
+def ini_value(path, section, name):
+    """The raw value of [section] name as the reference's INIReader gives it,
+    or None when it is not set."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    rc = lib().sbk_ini_value(str(path).encode(), section.encode(), name.encode(), buf, len(buf))
+    if rc == -1:      # SBK_EINVAL: not set
+        return None
+    _check(rc)
+    return buf.value.decode()
+
+
+def ini_names(path, section=None):
+    """Sections (section None) or the field names of a section, sorted."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    _check(lib().sbk_ini_names(str(path).encode(), None if section is None else section.encode(),
+                               buf, len(buf)))
+    return buf.value.decode().split("\n")[:-1]   # one per line, each ended by '\n' (names may be empty)
 
 
 def run_ini(path, table=None, precision=FP64_EXACT, time_mode=TIME_FROZEN, verbose=False):

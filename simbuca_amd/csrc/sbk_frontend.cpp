@@ -699,6 +699,44 @@ int make_folder(const std::string &prefix)
 
 }  // namespace
 
+namespace {
+
+int copy_out(const std::string &v, char *out, int len)
+{
+    if (!out || len <= (int)v.size()) return ffail(SBK_EINVAL, "the value needs %zu bytes", v.size() + 1);
+    std::memcpy(out, v.c_str(), v.size() + 1);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int sbk_ini_value(const char *ini_path, const char *section, const char *name, char *out,
+                             int len)
+{
+    if (!ini_path || !section || !name) return ffail(SBK_EINVAL, "bad arguments");
+    Ini ini;
+    if (!ini.parse(ini_path)) return ffail(SBK_EIO, "cannot read %s", ini_path);
+    auto it = ini.values.find(lower(std::string(section) + "=" + name));
+    if (it == ini.values.end()) return ffail(SBK_EINVAL, "[%s] %s is not set", section, name);
+    return copy_out(it->second, out, len);
+}
+
+extern "C" int sbk_ini_names(const char *ini_path, const char *section, char *out, int len)
+{
+    if (!ini_path) return ffail(SBK_EINVAL, "bad arguments");
+    Ini ini;
+    if (!ini.parse(ini_path)) return ffail(SBK_EIO, "cannot read %s", ini_path);
+    std::string v;
+    if (!section) {
+        for (const std::string &x : ini.sections) v += x + "\n";
+    } else {
+        auto it = ini.fields.find(lower(section));
+        if (it != ini.fields.end())
+            for (const std::string &x : it->second) v += x + "\n";
+    }
+    return copy_out(v, out, len);
+}
+
 extern "C" int sbk_ini_describe(const char *ini_path, const sbk_mass_table *table, char *json, int len)
 {
     Plan P;

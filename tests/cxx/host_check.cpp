@@ -174,7 +174,16 @@ static void check_ini(const std::string &dir, int argc, char **argv)
     for (const std::string &c : cases) {
         write_file(dir + "/c.ini", c);
         (void)sbk_ini_describe((dir + "/c.ini").c_str(), nullptr, json.data(), (int)json.size());
+        (void)sbk_ini_value((dir + "/c.ini").c_str(), "cloud", "NParticles", json.data(), (int)json.size());
+        (void)sbk_ini_names((dir + "/c.ini").c_str(), nullptr, json.data(), (int)json.size());
+        (void)sbk_ini_names((dir + "/c.ini").c_str(), "composition", json.data(), (int)json.size());
     }
+    char tiny[4];
+    write_file(dir + "/c.ini", base);
+    CHECK(sbk_ini_value((dir + "/c.ini").c_str(), "cloud", "NParticles", tiny, sizeof(tiny)) == 0);
+    CHECK(sbk_ini_value((dir + "/c.ini").c_str(), "output", "prefix", tiny, 1) != 0);
+    CHECK(sbk_ini_value((dir + "/c.ini").c_str(), "cloud", "nothing", tiny, sizeof(tiny)) != 0);
+    CHECK(sbk_ini_names((dir + "/c.ini").c_str(), nullptr, tiny, sizeof(tiny)) != 0);
     CHECK(sbk_ini_describe((dir + "/missing.ini").c_str(), nullptr, json.data(), (int)json.size()) != 0);
 }
 

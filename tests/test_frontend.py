@@ -424,6 +424,46 @@ def test_ini_syntax(tmp_path):
     assert d["skipped_line"] == 10
 
 
+INI_EDGE_CASES = [
+    "\ufeff[cloud]\nNParticles = 12\n",
+    "; comment\n# comment\n   ; indented comment\n[cloud]\nsx = 1 ; inline\nsy = 2;not inline\n"
+    "sz : 3\nx0:4\n",
+    "[cloud]\nNParticles = 10\n  temp = 0.5\n\tmore\n[ode]\n  method = DP\n",
+    "[cloud]\nsx = 1\nsx = 2\nSX = 3\nNParticles = 4\nnparticles = 5\n",
+    "[Cloud]\nsx = 1\n[cloud]\nsy = 2\n[CLOUD]\nsx = 9\n",
+    "[cloud]\nsx = 0.002" + " " * 189 + "sy = 0.007\n",
+    "[cloud]\nsx = 0.002" + " " * 195 + "sy = 0.007\n",
+    "[cloud]\n" + "k" * 60 + " = 1\n  continued\n",
+    "[" + "s" * 60 + "]\nx = 1\n",
+    "[cloud]\r\nsx = 1\r\nsy = 2 \r\n",
+    "[cloud]\n= nameless\nempty =\na = b = c\n   \n\t\n",
+    "[cloud]\nno separator\n[unclosed\nsx = 5\n[ok]\nx=1\n",
+    "x = before any section\n[cloud]\nsx = 1\n",
+    "[cloud]\nsx = 1\n[cloud ; comment]\nsy = 2\n[cloud]x = 3\n",
+]
+INI_QUERIES = [("cloud", "NParticles"), ("cloud", "temp"), ("cloud", "sx"), ("cloud", "sy"),
+               ("cloud", "sz"), ("cloud", "x0"), ("Cloud", "SX"), ("ode", "method"),
+               ("cloud", "k" * 60), ("cloud", "k" * 49), ("s" * 49, "x"), ("s" * 60, "x"),
+               ("cloud", ""), ("cloud", "empty"), ("cloud", "a"), ("ok", "x"), ("", "x"),
+               ("cloud", "no separator"), ("cloud ; comment", "sy"), ("cloud", "x"),
+               ("cloud ", "sy")]
+
+
+@have_refsim
+@pytest.mark.parametrize("k", range(len(INI_EDGE_CASES)))
+def test_ini_reader_matches_reference(tmp_path, k):
+    """The reader against the reference's own INIReader (bin/ini.c and
+    bin/inireader.cpp, compiled into oracle/_ref/ref_init) on edge cases of
+    the format: same sections, same field sets, same Get() values."""
+    path = tmp_path / "edge.ini"
+    path.write_bytes(INI_EDGE_CASES[k].encode("utf-8"))
+    ref_sections, ref = O.ref_ini(path, INI_QUERIES)
+    assert S.ini_names(path) == ref_sections
+    for (sec, name), (fields, value) in zip(INI_QUERIES, ref):
+        assert S.ini_names(path, sec) == fields, (sec, name)
+        assert S.ini_value(path, sec, name) == value, (sec, name)
+
+
 def test_ini_long_lines_and_sections(tmp_path):
     # inih reads 199-character chunks: the rest of a long line is a line of its own ...
     long = "sx = 0.002" + " " * 189 + "sy = 0.007\n"
