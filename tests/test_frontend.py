@@ -155,6 +155,39 @@ def test_masses_match_live_reference():
             assert t.mass(n) == m, n
 
 
+@have_refsim
+@have_tables
+def test_masses_every_table_row_matches_reference():
+    """Every name in both tables (AME names from the fixed columns, PDG names
+    from the particle lines) looked up here and by the reference's own
+    IonTable/PDGTable: the same table, the same mass, or absent from both."""
+    names = set()
+    for line in open(AME, encoding="latin-1"):
+        line = line.rstrip("\n")
+        if len(line) > 115 and len(line) != 124:
+            line = line.replace("#", " ")
+            el = line[20:22]
+            el = el.replace(" ", "", 1)
+            names.add("%d%s" % (int(line[16:19]), el))
+    for line in open(PDG, encoding="latin-1"):
+        w = line.split()
+        if len(w) >= 3 and not line.startswith("#") and w[0].isdigit() and w[2].lstrip("-").isdigit():
+            names.add(w[1])
+    names = sorted(n for n in names if n and " " not in n)
+    assert len(names) > 3000
+    ref = {}
+    for k in range(0, len(names), 500):
+        ref.update(O.ref_masses(names[k:k + 500]))
+    t = S.MassTable(AME, PDG)
+    for n in names:
+        kind, m = ref[n]
+        if kind == "none":
+            with pytest.raises(S.SimbucaError):
+                t.mass(n)
+        else:
+            assert t.mass(n) == m, n
+
+
 def test_species_names():
     # charge suffixes (bin/initialization.cpp:339-355) and the leading-digit mass
     assert S.parse_species("133Xx+") == (133.0, 1)
