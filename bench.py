@@ -127,6 +127,28 @@ def run_engine(S, w, precision, steps, warmup, timing, use_graph=True, tune=None
     return e, dt, st, status
 
 
+def cunbody_abi(S, w, calls=3):
+    """The drop-in for the reference's CUNBODY backend as force() calls it
+    (bin/cunbody.cpp:65-66): host FP64 arrays in and out, mj = 1,
+    eps2 = (double)1e-23f, ni = nj = N; milliseconds per cunbody1_force call
+    (H2D, kernel, reduction, D2H), FP64 and mixed."""
+    c = w.clouds[0]
+    xj = np.ascontiguousarray(c.pos)
+    xi = xj.copy()
+    mj = np.ones(c.n)
+    eps2 = float(np.float32(1e-23))
+    out = {"n": c.n}
+    for p in ("fp64", "mixed"):
+        S.cunbody1_force(xj, mj, xi, eps2, precision=p)          # buffers and first launch
+        device_sync()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            S.cunbody1_force(xj, mj, xi, eps2, precision=p)
+        device_sync()
+        out[f"{p}_ms_per_call"] = 1e3 * (time.perf_counter() - t0) / calls
+    return out
+
+
 def run_to_end(S, w, precision, tune=None, pieces=20):
     """The whole simulated interval of the workload (move_particles to t_end),
     advanced in `pieces` equal intervals with a progress line after each."""
@@ -450,6 +472,9 @@ def main():
     e64.close()
     log(f"fp64: {fp64}")
 
+    cun = cunbody_abi(S, w)
+    log(f"cunbody1_force ABI {cun}")
+
     traj = inv = None
     if not args.quick:
         traj = trajectory_deviation(S, w, tune, tune64)
@@ -505,6 +530,7 @@ def main():
         "fp64": fp64,
         "mixed_vs_fp64_ion_steps": ion_steps / fp64["ion_steps_per_s"] if fp64["ion_steps_per_s"] else None,
         "force_error": ferr,
+        "cunbody_abi": cun,
         "trajectory_deviation": traj,
         "invariant_drift": inv,
         "roofline": roofline,
