@@ -33,6 +33,8 @@ struct CunbodyState {
     int ni = 0, nj = 0;
     int precision = SBK_FP64;
     int cus = 256;
+    int items_key[6] = {-1, -1, -1, -1, -1, -1};   // (ni, nj, tpt, q, precision, unit) of the device list
+    std::vector<float> h_mf;                // mj as FP32, kept until the next call's copy
 };
 
 CunbodyState g;
@@ -68,8 +70,9 @@ void ensure(int ni, int nj)
             (void)hipStreamDestroy(g.stream);
             (void)hipSetDevice(dev);
         }
-        g = CunbodyState{g.device, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, 0, 0, 0,
-                         0, 0, g.precision, 256};
+        CunbodyState fresh;
+        fresh.precision = g.precision;
+        g = fresh;
         g.device = dev;
         hipDeviceProp_t p;
         CHK(hipGetDeviceProperties(&p, dev));
@@ -90,17 +93,27 @@ void ensure(int ni, int nj)
     }
 }
 
-void build_items(int ni, int nj, int tpt, int q)
+void build_items(int ni, int nj, int tpt, int q, int unit)
 {
+    const int key[6] = {ni, nj, tpt, q, g.precision, unit};
+    if (std::equal(key, key + 6, g.items_key)) return;   // the list on the device fits
     const int blk = sbk::kBlock / q * tpt;
     const int ib = (ni + blk - 1) / blk;
     int S = 1;
     if (g.precision != SBK_FP64_EXACT) {
-        const long long target = 8LL * g.cus;
-        S = (int)std::max(1LL, std::min<long long>((target + ib - 1) / ib, std::max(1, nj / 1024)));
+        // the engine's automatic split: about eight rounds of resident
+        // work-groups, at least 512 sources per slice (64 for sets too small
+        // to fill the device otherwise)
+        const long long capacity =
+            (long long)sbk::coulomb_blocks_per_cu(g.precision, tpt, unit, q, 0) * g.cus;
+        const long long target = 8 * capacity;
+        const long long cap = std::max(1, nj / ((long long)ib * 8 < capacity ? 64 : 512));
+        S = (int)std::max(1LL, std::min<long long>((target + ib / 2) / ib, cap));
     }
+    // slices of whole LDS tiles for all q wave groups, as the engine's
+    const int quantum = sbk::kSubTile * q;
     int chunk = (nj + S - 1) / S;
-    chunk = ((chunk + sbk::kSubTile - 1) / sbk::kSubTile) * sbk::kSubTile;
+    chunk = ((chunk + quantum - 1) / quantum) * quantum;
     S = std::max(1, (nj + chunk - 1) / chunk);
     std::vector<sbk::CoulombItem> items;
     for (int s = 0; s < S; ++s)
@@ -123,6 +136,7 @@ void build_items(int ni, int nj, int tpt, int q)
     CHK(hipMemcpyAsync(g.d_items, items.data(), sizeof(sbk::CoulombItem) * items.size(),
                        hipMemcpyHostToDevice, g.stream));
     CHK(hipStreamSynchronize(g.stream));   // `items` is pageable and goes out of scope
+    std::copy(key, key + 6, g.items_key);
 }
 
 double *partials = nullptr;
@@ -161,6 +175,9 @@ void cunbody1_force_begin(double xj[][3], double mj[], double xi[][3], double ep
         std::exit(EXIT_FAILURE);
     }
     ensure(std::max(ni, 1), std::max(nj, 1));
+    // a begin without its end: let that call's copies finish before the
+    // staging buffers are refilled
+    if (g.pending) CHK(hipStreamSynchronize(g.stream));
     g.ni = ni;
     g.nj = nj;
     g.pending = 1;
@@ -178,12 +195,19 @@ void cunbody1_force_begin(double xj[][3], double mj[], double xi[][3], double ep
     double *dx = g.d_buf, *dj = g.d_buf + 3 * ci, *dm = dj + 3 * cj, *dai = dm + cj;
     CHK(hipMemcpyAsync(dx, hx, sizeof(double) * 3 * ci, hipMemcpyHostToDevice, g.stream));
     CHK(hipMemcpyAsync(dj, hj, sizeof(double) * 4 * cj, hipMemcpyHostToDevice, g.stream));
-    std::vector<float> mf(nj);
-    for (int j = 0; j < nj; ++j) mf[j] = (float)mj[j];
-    CHK(hipMemcpyAsync(g.d_mf, mf.data(), sizeof(float) * nj, hipMemcpyHostToDevice, g.stream));
-    const int tpt = (g.precision == SBK_FP64_EXACT) ? 1 : 2;
-    const int q = (g.precision == SBK_FP64_EXACT) ? 1 : 4;
-    build_items(ni, nj, tpt, q);   // synchronizes: mf is consumed before it goes out of scope
+    // the FP32 weights come from a buffer that lives until the next call,
+    // which follows the end of this one (cunbody1_force_end synchronizes)
+    g.h_mf.resize(nj);
+    for (int j = 0; j < nj; ++j) g.h_mf[j] = (float)mj[j];
+    CHK(hipMemcpyAsync(g.d_mf, g.h_mf.data(), sizeof(float) * nj, hipMemcpyHostToDevice, g.stream));
+    // targets per thread by size, as the engine's automatic rule: 4 for
+    // large sets, 2 when the pairs fill the device, 1 for small sets
+    const bool exact = g.precision == SBK_FP64_EXACT;
+    const int tpt = exact ? 1 : (ni >= 32768 ? 4 : ((double)ni * nj >= 2.5e8 ? 2 : 1));
+    const int q = exact ? 1 : 4;
+    int unit = 1;
+    for (int j = 0; j < nj && unit; ++j) unit = mj[j] == 1.0;
+    build_items(ni, nj, tpt, q, unit);
     const int blk = sbk::kBlock / q * tpt;
     const int S = g.nitems / ((ni + blk - 1) / blk);   // items = slices x target blocks
     const long long stride = ci;
@@ -207,8 +231,7 @@ void cunbody1_force_begin(double xj[][3], double mj[], double xi[][3], double ep
     a.items = g.d_items;
     a.nitems = g.nitems;
     a.self_skip = 0;
-    a.unit_weights = 1;
-    for (int j = 0; j < nj && a.unit_weights; ++j) a.unit_weights = mj[j] == 1.0;
+    a.unit_weights = unit;
     a.state = nullptr;
     a.eps_d = eps2;
     a.eps_f = std::max((float)eps2, 1e-23f);   // FP32 needs eps >= 1e-23 to keep r=0 finite
