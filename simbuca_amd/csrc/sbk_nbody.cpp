@@ -82,8 +82,10 @@ void build_items(int n)
     const long long capacity =
         (long long)sbk::coulomb_blocks_per_cu(sbk::KP_MIXED, kTpt, 1, kGroups, 0) * g.cus;
     int S = (int)std::max(1LL, std::min<long long>((8 * capacity + ib / 2) / ib, std::max(1, n / 512)));
+    // slices of whole LDS tiles for all wave groups
+    const int quantum = sbk::kSubTile * kGroups;
     int chunk = (n + S - 1) / S;
-    chunk = ((chunk + sbk::kSubTile - 1) / sbk::kSubTile) * sbk::kSubTile;
+    chunk = ((chunk + quantum - 1) / quantum) * quantum;
     S = (n + chunk - 1) / chunk;
     std::vector<sbk::CoulombItem> items;
     for (int s = 0; s < S; ++s)
