@@ -635,6 +635,24 @@ def test_mixed_short_horizon_deviation(rng):
     assert dev < 1e-9 * np.abs(out["fp64"][0]).max()
 
 
+def test_coulomb_at_one_million_ions():
+    """2^20 ions, beyond every benchmark set-up (index arithmetic of items,
+    slices and partial sums at scale): the mixed sum, spatially ordered with
+    local coordinates, against the oracle's FP64 rows on sampled targets."""
+    n = 1 << 20
+    c = synth.gaussian_cloud(n, 3e-3, 300.0, species=((133.0, 0.7), (85.0, 0.3)), seed=9)
+    c.fields = synth.penning_fields()
+    print(f"Coulomb at {n} ions", flush=True)
+    with S.Engine(precision="mixed") as e:
+        e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
+        e.set_launch(4, 4, 0, 1)
+        f = e.eval_coulomb(0, "mixed")
+    rows = np.random.Generator(np.random.PCG64(20)).choice(n, 48, replace=False).astype(np.int32)
+    ref = O.coulomb_rows(c.pos, c.mass, c.charge, c.fields.coulomb_scale, rows)
+    assert _rows_error(f[rows], ref) <= 1e-6
+    assert np.all(np.isfinite(f))
+
+
 @pytest.mark.parametrize("index,steps", [(3, 20), (5, 4)])
 def test_full_size_short_horizon(index, steps):
     """The 65,536-ion and 262,144-ion set-ups at full size: fixed steps of the
