@@ -104,11 +104,12 @@ __global__ __launch_bounds__(kBlock) void coulomb_fp64_exact(CoulombArgs a)
 // every separation from it is at least kNearFactor h long, and the FP32
 // difference of two offsets has an error of at most
 // 2^-24 (2 + 2/kNearFactor) |r|, against 2^-24 |r| for the FP64-formed
-// separation.  A 64-source sub-tile whose sources are all far (one ballot
-// of the wave that staged it) takes the offset path, 3 instead of 6 vector
-// instructions per pair; the others keep the FP64-formed separation.  With
-// ions in a spatial order most sub-tiles are far; in any other order nearly
-// none is, so the accuracy never depends on the order, only the speed does.
+// separation.  The wave that stages a 64-source sub-tile stores its far
+// sources first (a ballot and a lane count), so each sub-tile runs its far
+// sources on the offset path, 3 instead of 6 vector instructions per pair,
+// and the rest on the FP64-formed separation.  With ions in a spatial order
+// most sources are far; in any other order nearly none is, so the accuracy
+// never depends on the order, only the speed does.
 // The self pair has r = 0 and contributes exactly 0 through the eps term.
 // ---------------------------------------------------------------------------
 constexpr float kNearFactor = 0.25f;
@@ -213,20 +214,35 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
         const int cnt = min(kBlock, it.j1 - t0);
         __syncthreads();
         if (LOC) {
-            // offsets; wave w stages sub-tile w and decides whether it is far
-            bool far_src = true;
+            // offsets; wave w stages sub-tile w, tests each source and stores
+            // the sub-tile's far sources first: a far source at its rank among
+            // the far lanes below it, a near one after all far ones.  Valid
+            // lanes are a prefix of the wave, so the valid sources fill
+            // [0, valid count) of the sub-tile.
+            bool far_src = false;
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            float4 s4 = make_float4(0.f, 0.f, 0.f, 1.f);
             if (tid < cnt) {
-                const double sx = a.sx[t0 + tid], sy = a.sy[t0 + tid], sz = a.sz[t0 + tid];
-                lx[tid] = sx;
-                ly[tid] = sy;
-                lz[tid] = sz;
-                const float4 s4 = make_float4((float)(sx - ox), (float)(sy - oy), (float)(sz - oz),
-                                              UNIT ? 1.f : a.sqf[t0 + tid]);
-                lf[tid] = s4;
+                sx = a.sx[t0 + tid];
+                sy = a.sy[t0 + tid];
+                sz = a.sz[t0 + tid];
+                s4 = make_float4((float)(sx - ox), (float)(sy - oy), (float)(sz - oz),
+                                 UNIT ? 1.f : a.sqf[t0 + tid]);
                 far_src = fmaf(s4.x, s4.x, fmaf(s4.y, s4.y, s4.z * s4.z)) >= far2;
             }
-            const unsigned long long all_far = __ballot(far_src);
-            if ((tid & 63) == 0) lfar[tid >> 6] = all_far == ~0ull ? 1 : 0;
+            const unsigned long long far_mask = __ballot(far_src);
+            const int nfar = __popcll(far_mask);
+            const int lane64 = tid & 63;
+            const int far_below = (int)__builtin_amdgcn_mbcnt_hi(
+                (unsigned)(far_mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)far_mask, 0u));
+            const int slot = (tid & ~63) + (far_src ? far_below : nfar + (lane64 - far_below));
+            if (tid < cnt) {
+                lx[slot] = sx;
+                ly[slot] = sy;
+                lz[slot] = sz;
+                lf[slot] = s4;
+            }
+            if (lane64 == 0) lfar[tid >> 6] = nfar;
         } else if (tid < cnt) {
             lx[tid] = a.sx[t0 + tid];
             ly[tid] = a.sy[t0 + tid];
@@ -287,31 +303,31 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
                         az[r] = fmaf(dz, w, az[r]);
                     }
                 };
-                auto run = [&](auto far_tag) {
+                // sources [j0, j1) of the sub-tile on one path
+                auto run = [&](int j0, int j1, auto far_tag) {
                     constexpr bool FAR = decltype(far_tag)::value;
-                    if (m == kSubTile) {
-                        if constexpr (FAR || !LOC) {
-#pragma unroll 8
-                            for (int j = 0; j < kSubTile; ++j) body(jj + j, far_tag);
-                        } else {
-                            // the minority path of LOC: a shorter unroll keeps the
-                            // kernel's register count at the far path's
-#pragma unroll 2
-                            for (int j = 0; j < kSubTile; ++j) body(jj + j, far_tag);
-                        }
+                    // the near path of LOC is the minority: a shorter unroll
+                    // keeps the kernel's register count at the far path's
+                    constexpr int U = (FAR || !LOC) ? 8 : 2;
+                    if (j1 - j0 == kSubTile) {
+#pragma unroll U
+                        for (int j = 0; j < kSubTile; ++j) body(j0 + j, far_tag);
                     } else {
-                        for (int j = 0; j < m; ++j) body(jj + j, far_tag);
+                        int j = j0;
+                        for (; j + U <= j1; j += U) {
+#pragma unroll
+                            for (int k = 0; k < U; ++k) body(j + k, far_tag);
+                        }
+                        for (; j < j1; ++j) body(j, far_tag);
                     }
                 };
-                int far = 0;
-                if (LOC) far = __builtin_amdgcn_readfirstlane(lfar[jj / kSubTile]);
                 if constexpr (LOC) {
-                    if (far)
-                        run(std::true_type{});
-                    else
-                        run(std::false_type{});
+                    // far sources first (nfar <= m: both count the sub-tile's valid sources)
+                    const int nfar = __builtin_amdgcn_readfirstlane(lfar[jj / kSubTile]);
+                    if (nfar > 0) run(jj, jj + nfar, std::true_type{});
+                    if (nfar < m) run(jj + nfar, jj + m, std::false_type{});
                 } else {
-                    run(std::false_type{});
+                    run(jj, jj + m, std::false_type{});
                 }
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
