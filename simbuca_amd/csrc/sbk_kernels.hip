@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
     __shared__ float lqf[(LOC || UNIT) ? 1 : kBlock];
     __shared__ double lqd[(MODE == KP_FP64) ? kBlock : 1];
     __shared__ float4 lf[LOC ? kBlock : 1];                 // LOC: (x_s - o, weight)
-    __shared__ int lfar[LOC ? kBlock / kSubTile : 1];       // LOC: sub-tile is far
+    __shared__ int lfar[LOC ? kBlock / kSubTile : 1];       // LOC: far sources per sub-tile
 
     const CoulombItem it = a.items[xcd_item(blockIdx.x, gridDim.x)];
     if (a.state && (!a.state[it.cloud].active || (a.skip_valid_k1 && a.state[it.cloud].k1_valid)))
