@@ -28,6 +28,8 @@ struct CunbodyState {
     double *h_buf = nullptr;        // pinned staging of the same layout
     sbk::CoulombItem *d_items = nullptr;
     int cap_items = 0;
+    double *d_cp = nullptr;         // slice partials [slices][3][cap_i]
+    long long cap_cp = 0;
     int nitems = 0;
     int pending = 0;
     int ni = 0, nj = 0;
@@ -66,6 +68,7 @@ void ensure(int ni, int nj)
             (void)hipFree(g.d_buf);
             (void)hipFree(g.d_mf);
             (void)hipFree(g.d_items);
+            (void)hipFree(g.d_cp);
             (void)hipHostFree(g.h_buf);
             (void)hipStreamDestroy(g.stream);
             (void)hipSetDevice(dev);
@@ -139,9 +142,6 @@ void build_items(int ni, int nj, int tpt, int q, int unit)
     std::copy(key, key + 6, g.items_key);
 }
 
-double *partials = nullptr;
-long long partials_cap = 0;
-
 __global__ void reduce_slices(const double *cp, long long stride, int slices, int ni, double *ai)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -186,6 +186,12 @@ void cunbody1_force_begin(double xj[][3], double mj[], double xi[][3], double ep
     double *hx = g.h_buf;                 // xi [3][ci]
     double *hj = g.h_buf + 3 * ci;        // xj [3][cj]
     double *hm = hj + 3 * cj;             // mj [cj]
+    double *hai = hm + cj;                // ai [3][ni], filled by the D2H copy
+    if (nj == 0) {
+        // no sources: every sum is empty
+        std::fill(hai, hai + 3 * (size_t)ni, 0.0);
+        return;
+    }
     for (int i = 0; i < ni; ++i)
         for (int k = 0; k < 3; ++k) hx[k * ci + i] = xi[i][k];
     for (int j = 0; j < nj; ++j) {
@@ -211,10 +217,11 @@ void cunbody1_force_begin(double xj[][3], double mj[], double xi[][3], double ep
     const int blk = sbk::kBlock / q * tpt;
     const int S = g.nitems / ((ni + blk - 1) / blk);   // items = slices x target blocks
     const long long stride = ci;
-    if ((long long)S * 3 * stride > partials_cap) {
-        (void)hipFree(partials);
-        partials_cap = (long long)S * 3 * stride;
-        CHK(hipMalloc(&partials, sizeof(double) * partials_cap));
+    if ((long long)S * 3 * stride > g.cap_cp) {
+        CHK(hipStreamSynchronize(g.stream));   // earlier launches may still write the old buffer
+        (void)hipFree(g.d_cp);
+        g.cap_cp = (long long)S * 3 * stride;
+        CHK(hipMalloc(&g.d_cp, sizeof(double) * g.cap_cp));
     }
     sbk::CoulombArgs a{};
     a.tx = dx;
@@ -226,7 +233,7 @@ void cunbody1_force_begin(double xj[][3], double mj[], double xi[][3], double ep
     a.sqd = dm;
     a.sqf = g.d_mf;
     a.kc = nullptr;
-    a.out = partials;
+    a.out = g.d_cp;
     a.stride = stride;
     a.items = g.d_items;
     a.nitems = g.nitems;
@@ -237,9 +244,8 @@ void cunbody1_force_begin(double xj[][3], double mj[], double xi[][3], double ep
     a.eps_f = std::max((float)eps2, 1e-23f);   // FP32 needs eps >= 1e-23 to keep r=0 finite
     if (sbk::launch_coulomb(g.precision, tpt, q, a, g.stream)) die("Coulomb kernel launch", hipGetLastError());
     hipLaunchKernelGGL(reduce_slices, dim3((ni + 255) / 256), dim3(256), 0, g.stream,
-                       (const double *)partials, stride, S, ni, dai);
+                       (const double *)g.d_cp, stride, S, ni, dai);
     CHK(hipGetLastError());
-    double *hai = hm + cj;   // staging for ai [3][ni]
     CHK(hipMemcpyAsync(hai, dai, sizeof(double) * 3 * ni, hipMemcpyDeviceToHost, g.stream));
 }
 

@@ -728,6 +728,24 @@ def test_cunbody_abi_fp64_and_mixed(rng):
     assert _rows_error(S.cunbody1_force(xj, ones, xi, 1e-20, precision="mixed_kahan"), ref1) < 1e-6
 
 
+def test_cunbody_abi_empty_sets_and_changing_sizes(rng):
+    # no sources: every sum is empty, in every precision
+    xi = rng.normal(0, 1e-3, (10, 3))
+    for p in ("fp64_exact", "fp64", "mixed", "mixed_kahan"):
+        ai = S.cunbody1_force(np.zeros((0, 3)), np.zeros(0), xi, 1e-20, precision=p)
+        assert np.array_equal(ai, np.zeros((10, 3))), p
+    # no targets
+    assert S.cunbody1_force(xi, np.ones(10), np.zeros((0, 3)), 1e-20).shape == (0, 3)
+    # sizes that grow, shrink and grow again: the cached device buffers, work
+    # lists and slice partials are reused or regrown between calls
+    for ni, nj in ((100, 300), (5000, 7000), (50, 20), (9000, 300), (300, 12000), (64, 64)):
+        xj = rng.normal(0, 1e-3, (nj, 3)); xs = rng.normal(0, 1e-3, (ni, 3))
+        mj = rng.uniform(0.5, 2.0, nj)
+        ref = O.cunbody_force(xj, mj, xs, 1e-20)
+        assert _rows_error(S.cunbody1_force(xj, mj, xs, 1e-20, precision="fp64"), ref) < 1e-13, (ni, nj)
+        assert _rows_error(S.cunbody1_force(xj, mj, xs, 1e-20, precision="mixed"), ref) < 1e-6, (ni, nj)
+
+
 # ---------------------------------------------------------------------------
 # NBODY ABI (initialize / simulation_Async / finalize)
 # ---------------------------------------------------------------------------
