@@ -49,9 +49,12 @@ extern "C" {
  * ni = nj = N (bin/cunbody.cpp:31-35,66) and scales the result by
  * -coulomb_scale e^2 ke / m_i itself (bin/force.cpp:87-93).
  * _begin enqueues the work and returns; _end (same arguments) blocks until
- * ai is written.  cunbody1_force does both.  Errors abort the process with
- * a message, as the reference backends do (bin/nbody_kernel.cu:37-47);
- * the message is also kept for sbk_last_error().
+ * ai is written.  cunbody1_force does both.  nj = 0 gives ai = 0; ni = 0
+ * writes nothing.  Errors print a message to stderr and end the process
+ * with EXIT_FAILURE, as the reference backends do
+ * (bin/nbody_kernel.cu:37-47).  The device buffers are one process-wide
+ * set on the device current at the call, so calls come from one host
+ * thread at a time, as the reference's force() makes them.
  * ---------------------------------------------------------------------- */
 void cunbody1_force(double xj[][3], double mj[], double xi[][3], double eps2, double ai[][3],
                     int ni, int nj);
@@ -77,6 +80,9 @@ int sbk_cunbody_set_precision(int precision);
  * on stream 0, so an event the caller records on stream 0 after
  * simulation_Async completes with it (bin/nbody.cpp:243-245,265-268).
  * p, q, ntiles are the reference's launch-table hints and are ignored.
+ * initialize() allocates for up to n_max bodies on the device current at
+ * that call; later calls use those buffers with the same device current.
+ * Errors print a message and end the process with EXIT_FAILURE.
  * ---------------------------------------------------------------------- */
 typedef struct sbk_float4 { float x, y, z, w; } sbk_float4;   /* layout of float4 */
 typedef struct sbk_float3 { float x, y, z; } sbk_float3;      /* layout of float3 */
