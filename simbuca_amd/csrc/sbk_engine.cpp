@@ -379,6 +379,22 @@ int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
             }
         }
     }
+    // Work-group b runs on XCD b % 8 and takes the b / 8-th item of that
+    // XCD's contiguous run (sbk::xcd_item); an XCD starts its work-groups
+    // in order.  Each run is therefore ordered costliest first (an item
+    // costs its source count: every lane loops over all of them), so that
+    // short items fill the last, partly empty round of the XCD; the runs
+    // themselves keep their clouds, which keeps each XCD's sources in its L2.
+    {
+        const int nb = (int)items.size(), qx = nb / 8, rx = nb % 8;
+        for (int x = 0; x < 8; ++x) {
+            const int b0 = x * qx + std::min(x, rx), cnt = qx + (x < rx ? 1 : 0);
+            std::stable_sort(items.begin() + b0, items.begin() + b0 + cnt,
+                             [](const sbk::CoulombItem &u, const sbk::CoulombItem &v) {
+                                 return u.j1 - u.j0 > v.j1 - v.j0;
+                             });
+        }
+    }
     L.nitems = (int)items.size();
     if (L.nitems > 0) {
         HIP_TRY(hipMalloc(&L.d_items, sizeof(sbk::CoulombItem) * items.size()));
