@@ -279,6 +279,35 @@ def test_local_coords_keep_the_accuracy(exact_65536):
     assert errs[1] <= 2.0 * errs[0], errs
 
 
+@pytest.mark.parametrize("precision", ["mixed", "mixed_kahan"])
+@pytest.mark.parametrize("unit", [True, False])
+@pytest.mark.parametrize("tpt,groups", [(1, 4), (2, 4), (4, 4), (2, 1)])
+def test_local_coords_close_pairs_take_the_fp64_separation(rng, precision, unit, tpt, groups):
+    # 16 pairs of ions 20 nm apart in a 5,003-ion cloud: each partner's pull
+    # outweighs the rest of the cloud about 1e5 times, so the per-ion error of
+    # these 32 ions is the error of one near pair.  Formed from FP32 offsets
+    # to a block origin some 1e-4 m away, that separation would err by about
+    # 1e-4 of itself; formed in FP64 it errs by about 1e-7, and the mixed
+    # kernel's softening (1e-23 m^2 added to r^2) moves it by 4e-8.  A source
+    # put on the wrong path, or a lost or doubled staging slot, fails this test.
+    n = 5003
+    pos = rng.normal(0, 5e-4, (n, 3))
+    pos[4000:4016] = pos[100:116] + np.array([2e-8, 0.0, 0.0])
+    close = np.r_[100:116, 4000:4016]
+    amu = np.full(n, 133.0)
+    ch = np.ones(n, np.int32) if unit else rng.choice(np.array([1, 2, 3], np.int32), n)
+    sf, _ = _both_fields(trap=0, coulomb=1)
+    with S.Engine(precision=precision) as e:
+        e.add_cloud(pos, np.zeros((n, 3)), O.mass_kg(amu), ch, sf)
+        exact = e.eval_coulomb(0, "fp64_exact")
+        for local in (1, 0):
+            e.set_launch(tpt, groups, 0, local)
+            a = e.eval_coulomb(0, precision)
+            assert _rows_error(a, exact) < 1e-6, local
+            per_ion = np.linalg.norm(a[close] - exact[close], axis=1) / np.linalg.norm(exact[close], axis=1)
+            assert per_ion.max() < 1e-6, (local, per_ion.max())
+
+
 @pytest.mark.parametrize("tpt", [1, 2, 4])
 @pytest.mark.parametrize("groups", [1, 4])
 @pytest.mark.parametrize("slices", [0, 1, 3, 7])
