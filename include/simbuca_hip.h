@@ -244,6 +244,28 @@ int  sbk_synchronize(sbk_engine *e);
  * whose ion order never matters for speed. */
 int  sbk_reorder(sbk_engine *e);
 
+/* The print_at_x detector of move_particles (bin/ion_fly.cpp:91-93,110-114)
+ * on the device: with enable != 0, every accepted step of `cloud` records
+ * each ion whose x lies strictly on one side of x_plane before the step and
+ * strictly on the other after it, with its state after the step.  The list
+ * lives on the device and holds *list_capacity entries (max(65,536, ions of
+ * the engine), at most 2^24); sbk_take_crossings empties it.  A run that
+ * records more between two takes fails with SBK_ESTATE (the extra entries
+ * are lost); at most one entry per ion and accepted step is recorded, so
+ * taking at least every list_capacity / n steps is enough.  enable = 0
+ * stops recording.  May be called before or after the first run. */
+int  sbk_set_crossing_plane(sbk_engine *e, int cloud, int enable, double x_plane,
+                            int *list_capacity);
+/* Takes the recorded crossings of every cloud, ordered by cloud, then step,
+ * then ion (the order the reference prints them in), and empties the list.
+ * Per entry: cloud, the caller's ion index, the cloud's accepted-step count
+ * after the step, the lifetime after the step, and pos/vel [3] after it.
+ * capacity = 0 with every output array NULL only sets *n to the number of
+ * entries held and keeps them; otherwise capacity must be >= that number.
+ * Any output array may be NULL. */
+int  sbk_take_crossings(sbk_engine *e, int capacity, int *n, int *cloud, int *ion,
+                        long long *step, double *t, double *pos, double *vel);
+
 int  sbk_get_state(sbk_engine *e, int cloud, double *pos, double *vel);
 int  sbk_get_status(sbk_engine *e, int cloud, sbk_cloud_status *st);
 int  sbk_get_stats(sbk_engine *e, sbk_stats *st);

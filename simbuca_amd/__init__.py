@@ -121,7 +121,7 @@ EXPORTS = [
     "initialize", "finalize", "simulation_Async", "simulation_Async_begin", "simulation_Async_end",
     "sbk_default_config", "sbk_default_fields", "sbk_create", "sbk_destroy", "sbk_add_cloud",
     "sbk_set_fields", "sbk_set_step", "sbk_advance", "sbk_advance_until", "sbk_step", "sbk_attempts",
-    "sbk_synchronize", "sbk_reorder", "sbk_get_state", "sbk_get_status", "sbk_get_stats", "sbk_reset_stats",
+    "sbk_synchronize", "sbk_reorder", "sbk_set_crossing_plane", "sbk_take_crossings", "sbk_get_state", "sbk_get_status", "sbk_get_stats", "sbk_reset_stats",
     "sbk_eval_force", "sbk_eval_coulomb", "sbk_set_launch", "sbk_time_coulomb", "sbk_autotune",
     "sbk_load_tuning", "sbk_spatial_order", "sbk_writer_open", "sbk_writer_particles",
     "sbk_writer_cloud", "sbk_writer_set_ion", "sbk_writer_x", "sbk_writer_close", "sbk_import", "sbk_init_cloud", "sbk_mass_table_open",
@@ -167,6 +167,8 @@ def lib():
         "sbk_attempts": (c_int, [vp, c_int]),
         "sbk_synchronize": (c_int, [vp]),
         "sbk_reorder": (c_int, [vp]),
+        "sbk_set_crossing_plane": (c_int, [vp, c_int, c_int, c_dbl, ip]),
+        "sbk_take_crossings": (c_int, [vp, c_int, ip, ip, ip, ctypes.POINTER(c_ll), dp, dp, dp]),
         "sbk_get_state": (c_int, [vp, c_int, dp, dp]),
         "sbk_get_status": (c_int, [vp, c_int, ctypes.POINTER(CloudStatus)]),
         "sbk_get_stats": (c_int, [vp, ctypes.POINTER(Stats)]),
@@ -499,6 +501,30 @@ class Engine:
 
     def reorder(self):
         _check(lib().sbk_reorder(self._h))
+
+    def set_crossing_plane(self, cloud, x_plane, enable=True):
+        """Records the ions of `cloud` that cross x = x_plane in each accepted
+        step (the reference's print_at_x); returns the device list's capacity."""
+        cap = ctypes.c_int()
+        _check(lib().sbk_set_crossing_plane(self._h, cloud, 1 if enable else 0, float(x_plane),
+                                            ctypes.byref(cap)))
+        return cap.value
+
+    def take_crossings(self):
+        """The recorded crossings ordered by cloud, step and ion (the list is
+        emptied): dict of arrays cloud, ion, step, t, pos (n, 3), vel (n, 3)."""
+        L = lib()
+        n = ctypes.c_int()
+        _check(L.sbk_take_crossings(self._h, 0, ctypes.byref(n), None, None, None, None, None, None))
+        k = n.value
+        out = {"cloud": np.zeros(k, np.int32), "ion": np.zeros(k, np.int32),
+               "step": np.zeros(k, np.int64), "t": np.zeros(k),
+               "pos": np.zeros((k, 3)), "vel": np.zeros((k, 3))}
+        _check(L.sbk_take_crossings(
+            self._h, max(k, 1), ctypes.byref(n), _ip(out["cloud"]), _ip(out["ion"]),
+            out["step"].ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _dp(out["t"]),
+            _dp(out["pos"]), _dp(out["vel"])))
+        return out
 
     def state(self, cloud=0):
         n = self.sizes[cloud]

@@ -71,6 +71,15 @@ struct HostCloud {
     std::vector<int> charge;
     std::vector<int> perm;                // slot -> caller's index (empty: identity)
     std::vector<double> swift;            // SWIFT samples, copied from the fields
+    int xcross_on = 0;                    // record crossings of the plane x = xcross
+    double xcross = 0.0;
+};
+
+// A crossing taken off the device list, with the ion's caller index.
+struct HostCrossing {
+    int cloud, ion;
+    long long step;
+    double t, p[3], v[3];
 };
 
 struct ItemList {
@@ -114,6 +123,12 @@ struct sbk_engine {
     long long attempts_at_order = 0;
     long long reorders = 0, reorders_at_reset = 0;
     sbk::EngineDev dev{};
+
+    // crossing list, allocated once a cloud records crossings
+    sbk::XCrossing *d_xc = nullptr;
+    int *d_xc_count = nullptr;
+    int xc_cap = 0;
+    std::vector<HostCrossing> xc_host;   // entries drained from the device
 
     // launch shape (tpt 0: automatic)
     int tpt = 0;
@@ -432,7 +447,73 @@ sbk::CloudParams cloud_params(const sbk_engine *e, size_t ci)
                        c.f.Uo != 0.0 || c.f.Uw != 0.0 || c.f.Eax != 0.0 || c.f.swift_n > 0;
     if (c.f.swift_n > 0 && e->d_swift) p.swift = e->d_swift + e->swift_off[ci];
     p.fsal = !c.f.ramp && (e->cfg.time_mode == SBK_TIME_STAGE || !timed);
+    p.xcross_on = c.xcross_on;
+    p.xcross = c.xcross;
     return p;
+}
+
+// Entries of the crossing list: one per ion of the engine, at least 65,536
+// and at most 2^24 (72 B each).
+int crossing_capacity(long long ntot)
+{
+    return (int)std::min<long long>(1LL << 24, std::max<long long>(65536, ntot));
+}
+
+// Allocates the crossing list once a cloud records crossings.  The attempt
+// graph holds the device descriptor by value, so it is rebuilt.  The stream
+// must be idle.
+int ensure_crossings(sbk_engine *e)
+{
+    if (e->d_xc) return 0;
+    bool any = false;
+    for (auto &c : e->clouds) any = any || c.xcross_on;
+    if (!any) return 0;
+    e->xc_cap = crossing_capacity(e->ntot);
+    HIP_TRY(hipMalloc(&e->d_xc, sizeof(sbk::XCrossing) * (size_t)e->xc_cap));
+    HIP_TRY(hipMalloc(&e->d_xc_count, sizeof(int)));
+    HIP_TRY(hipMemsetAsync(e->d_xc_count, 0, sizeof(int), e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->dev.xc = e->d_xc;
+    e->dev.xc_count = e->d_xc_count;
+    e->dev.xc_cap = e->xc_cap;
+    destroy_graph(e);
+    return 0;
+}
+
+// Moves the device list's entries to the host, with the caller indices of
+// the current ion order, and empties it.  The stream must be idle.
+int drain_crossings(sbk_engine *e)
+{
+    if (!e->d_xc) return 0;
+    int count = 0;
+    HIP_TRY(hipMemcpyAsync(&count, e->d_xc_count, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    const int kept = std::min(count, e->xc_cap);
+    if (kept > 0) {
+        std::vector<sbk::XCrossing> v(kept);
+        HIP_TRY(hipMemcpyAsync(v.data(), e->d_xc, sizeof(sbk::XCrossing) * (size_t)kept,
+                               hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        for (const sbk::XCrossing &x : v) {
+            const HostCloud &c = e->clouds[x.cloud];
+            HostCrossing h;
+            h.cloud = x.cloud;
+            h.ion = caller_index(c, x.slot - c.off);
+            h.step = x.step;
+            h.t = x.t;
+            for (int k = 0; k < 3; ++k) {
+                h.p[k] = x.p[k];
+                h.v[k] = x.v[k];
+            }
+            e->xc_host.push_back(h);
+        }
+    }
+    HIP_TRY(hipMemsetAsync(e->d_xc_count, 0, sizeof(int), e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (count > e->xc_cap)
+        return fail(SBK_ESTATE, "the crossing list overflowed (%d crossings, room for %d): take them "
+                                "more often", count, e->xc_cap);
+    return 0;
 }
 
 // Concatenates every cloud's SWIFT samples into one device buffer (grown as
@@ -598,6 +679,7 @@ int finalize(sbk_engine *e)
     d.adaptive = e->cfg.adaptive;
     d.rtol = e->cfg.rtol;
     d.atol = e->cfg.atol;
+    if (int rc2 = ensure_crossings(e)) return rc2;
     d.CP = nullptr;
     d.slices = nullptr;
     e->unit_charges = true;
@@ -739,6 +821,8 @@ int read_states(sbk_engine *e)
 int reorder(sbk_engine *e)
 {
     if (!e->spatial) return 0;
+    // recorded crossings name slots of the order that is about to change
+    if (int rc = drain_crossings(e)) return rc;
     const long long S = e->stride;
     const int n = e->ntot;
     std::vector<double> soa(3 * (size_t)S);
@@ -909,6 +993,8 @@ int sbk_destroy(sbk_engine *e)
     (void)hipFree(e->d_state);
     (void)hipFree(e->d_src);
     (void)hipFree(e->d_swift);
+    (void)hipFree(e->d_xc);
+    (void)hipFree(e->d_xc_count);
     if (e->h_state) (void)hipHostFree(e->h_state);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -1129,6 +1215,68 @@ int sbk_get_state(sbk_engine *e, int cloud, double *pos, double *vel)
             if (vel) vel[o + k] = buf[(3 + k) * (size_t)c.n + i];
         }
     }
+    return 0;
+}
+
+int sbk_set_crossing_plane(sbk_engine *e, int cloud, int enable, double x_plane,
+                           int *list_capacity)
+{
+    if (!e || cloud < 0 || cloud >= (int)e->clouds.size())
+        return fail(SBK_EINVAL, "bad cloud id");
+    if (enable && !std::isfinite(x_plane)) return fail(SBK_EINVAL, "the crossing plane must be finite");
+    HostCloud &c = e->clouds[cloud];
+    c.xcross_on = enable ? 1 : 0;
+    c.xcross = enable ? x_plane : 0.0;
+    if (e->finalized) {
+        HIP_TRY(hipSetDevice(e->device));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        if (int rc = ensure_crossings(e)) return rc;
+        if (int rc = upload_params(e)) return rc;
+    }
+    if (list_capacity) {
+        long long n = 0;
+        for (auto &h : e->clouds) n += h.n;
+        *list_capacity = e->d_xc ? e->xc_cap : crossing_capacity(n);
+    }
+    return 0;
+}
+
+int sbk_take_crossings(sbk_engine *e, int capacity, int *n, int *cloud, int *ion, long long *step,
+                       double *t, double *pos, double *vel)
+{
+    if (!e || !n) return fail(SBK_EINVAL, "null argument");
+    if (e->finalized && e->d_xc) {
+        HIP_TRY(hipSetDevice(e->device));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        if (int rc = drain_crossings(e)) return rc;
+    }
+    std::vector<HostCrossing> &v = e->xc_host;
+    // the reference prints a step's crossings in ion order (bin/ion_fly.cpp:110-114)
+    std::stable_sort(v.begin(), v.end(), [](const HostCrossing &a, const HostCrossing &b) {
+        if (a.cloud != b.cloud) return a.cloud < b.cloud;
+        if (a.step != b.step) return a.step < b.step;
+        return a.ion < b.ion;
+    });
+    const int count = (int)v.size();
+    if (capacity == 0 && !cloud && !ion && !step && !t && !pos && !vel) {
+        *n = count;   // size query: the entries stay
+        return 0;
+    }
+    if (capacity < count)
+        return fail(SBK_EINVAL, "capacity %d is below the %d crossings held", capacity, count);
+    for (int i = 0; i < count; ++i) {
+        const HostCrossing &h = v[i];
+        if (cloud) cloud[i] = h.cloud;
+        if (ion) ion[i] = h.ion;
+        if (step) step[i] = h.step;
+        if (t) t[i] = h.t;
+        for (int k = 0; k < 3; ++k) {
+            if (pos) pos[3 * (size_t)i + k] = h.p[k];
+            if (vel) vel[3 * (size_t)i + k] = h.v[k];
+        }
+    }
+    *n = count;
+    v.clear();
     return 0;
 }
 

@@ -888,6 +888,7 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
     cfg.adaptive = P.adaptive ? 1 : 0;
     cfg.h0 = P.timestep;
     double pending_h = 0.0;   // an ode_timestep setting made before the engine exists
+    int xc_cap = 0;   // entries of the engine's crossing list
     auto start_engine = [&]() -> int {
         if (eg.e) return 0;
         if (int rc = sbk_create(&cfg, &eg.e)) return rc;
@@ -896,6 +897,8 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
             return rc;
         if (pending_h > 0.0)
             if (int rc = sbk_set_step(eg.e, 0, pending_h)) return rc;
+        if (P.print_at_x)
+            if (int rc = sbk_set_crossing_plane(eg.e, 0, 1, P.print_x, &xc_cap)) return rc;
         return 0;
     };
 
@@ -911,14 +914,42 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
     // unless imported; the reference measures only its first operation from
     // there and the others from 0).  The engine stops at the first step past
     // the next print time (or at the end), which is the step the reference
-    // prints at.  With print_at_x it steps one step at a time and reports the
-    // ions whose x crossed print_x in that step.
+    // prints at.  With print_at_x the engine records, in every accepted step,
+    // the ions whose x crossed print_x (strictly, from either side), and they
+    // are written in step and ion order after each advance; an advance takes
+    // at most as many attempts as the crossing list has room for.
     double print_interval = P.print_interval;
     int nr_interval = 0;
     sbk_cloud_status st;
     std::memset(&st, 0, sizeof(st));
     st.lifetime = P.lifetime0;
-    std::vector<double> old_x(N);
+    std::vector<double> xpos, xvel;   // one crossing's state at its ion's slot
+    std::vector<int> xc_ion;
+    std::vector<double> xc_t, xc_pos, xc_vel;
+    auto write_crossings = [&]() -> int {
+        int n = 0;
+        if (int rc = sbk_take_crossings(eg.e, 0, &n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr))
+            return rc;
+        if (n == 0) return 0;
+        xc_ion.resize(n);
+        xc_t.resize(n);
+        xc_pos.resize(3 * (size_t)n);
+        xc_vel.resize(3 * (size_t)n);
+        if (int rc = sbk_take_crossings(eg.e, n, &n, nullptr, xc_ion.data(), nullptr, xc_t.data(),
+                                        xc_pos.data(), xc_vel.data()))
+            return rc;
+        xpos.assign(3 * (size_t)N, 0.0);
+        xvel.assign(3 * (size_t)N, 0.0);
+        for (int i = 0; i < n; ++i) {
+            const int k = xc_ion[i];
+            for (int c = 0; c < 3; ++c) {
+                xpos[3 * (size_t)k + c] = xc_pos[3 * (size_t)i + c];
+                xvel[3 * (size_t)k + c] = xc_vel[3 * (size_t)i + c];
+            }
+            if (int rc = sbk_writer_x(wg.w, k, xc_t[i], xpos.data(), xvel.data())) return rc;
+        }
+        return 0;
+    };
     for (const Operation &op : P.ops) {
         if (op.print_time) print_interval = op.print_time;
         if (op.timestep) {   // the reference sets h to the print step here (bin/operation.cpp:37-40)
@@ -960,7 +991,7 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
         while (st.lifetime < endtime) {
             const double t_print = print_interval * nr_interval;
             double target = st.lifetime;
-            if (!P.print_at_x && !(target - P.lifetime0 > t_print)) {
+            if (!(target - P.lifetime0 > t_print)) {
                 // the first lifetime with lifetime - t0 > t_print: the least
                 // such double, as x - t0 rounds monotonically in x
                 target = P.lifetime0 + t_print;
@@ -969,23 +1000,18 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
             }
             if (target <= st.lifetime) target = std::nextafter(st.lifetime, INFINITY);   // one step
             if (target > endtime) target = endtime;
-            if (P.print_at_x)
-                for (int i = 0; i < N; ++i) old_x[i] = pos[3 * (size_t)i];
-            if (int rc = sbk_advance_until(e, target, 1LL << 62)) return rc;
+            const long long max_attempts =
+                P.print_at_x ? std::max(1, xc_cap / std::max(N, 1)) : (1LL << 62);
+            if (int rc = sbk_advance_until(e, target, max_attempts)) return rc;
             if (int rc = sbk_get_status(e, 0, &st)) return rc;
+            if (P.print_at_x)
+                if (int rc = write_crossings()) return rc;
             const bool due = st.lifetime - P.lifetime0 > t_print;
-            if (due || P.print_at_x)
-                if (int rc = sbk_get_state(e, 0, pos.data(), vel.data())) return rc;
             if (due) {
+                if (int rc = sbk_get_state(e, 0, pos.data(), vel.data())) return rc;
                 if (int rc = print(st.lifetime)) return rc;
                 nr_interval++;
             }
-            if (P.print_at_x)
-                for (int i = 0; i < N; ++i) {
-                    const double x = pos[3 * (size_t)i];
-                    if ((old_x[i] < P.print_x && x > P.print_x) || (old_x[i] > P.print_x && x < P.print_x))
-                        if (int rc = sbk_writer_x(wg.w, i, st.lifetime, pos.data(), vel.data())) return rc;
-                }
         }
         if (g.ramp && st.steps > 0 && st.t_prev >= t_start) {
             // the trap keeps the values of the last step's start (bin/ion_fly.cpp:95-99)

@@ -54,7 +54,17 @@ struct CloudParams {
     int blk0, nblk;           // this cloud's range in the ion-block table
     int fsal;                 // an accepted attempt's stage-7 derivative is exactly the
                               // next step's stage-1 derivative (same state and time)
-    int pad2;
+    int xcross_on;            // record the ions whose x crosses xcross in an accepted step
+    double xcross;
+};
+
+// One ion that crossed the plane x = CloudParams::xcross in an accepted step
+// (bin/ion_fly.cpp:110-114), with its state after the step.
+struct XCrossing {
+    long long step;           // the cloud's accepted-step count after the step
+    int cloud, slot;          // slot: the ion's index in the engine's arrays
+    double t;                 // lifetime after the step
+    double p[3], v[3];
 };
 
 struct IonBlock {
@@ -114,6 +124,10 @@ struct EngineDev {
     int ntot, nclouds;
     int time_mode, adaptive;
     double rtol, atol;
+    XCrossing *xc;                // crossing list (nullptr: no cloud records crossings)
+    int *xc_count;                // entries written since the list was last emptied
+    int xc_cap;                   // entries the list holds
+    int pad_xc;
 };
 
 // sets the message sbk_last_error() returns; returns code (sbk_engine.cpp)

@@ -712,13 +712,36 @@ __global__ __launch_bounds__(256) void controller_kernel(EngineDev d)
 
 // pos <- pos2, vel <- vel2 for clouds whose attempt was accepted
 // (bin/ode.cpp:488-489).
+// With a crossing plane, an ion whose x lies strictly on one side of it
+// before the step and strictly on the other after it (bin/ion_fly.cpp:91-93,
+// 110-114) is appended to the crossing list with its new state.
 __global__ __launch_bounds__(256) void commit_kernel(EngineDev d)
 {
     const IonBlock blk = d.blocks[blockIdx.x];
-    if (!d.state[blk.cloud].accepted_last) return;
+    const CloudState &cs = d.state[blk.cloud];
+    if (!cs.accepted_last) return;
     const long long g = blk.start + threadIdx.x;
     if (g >= blk.end) return;
-    const bool fsal = d.params[blk.cloud].fsal;
+    const CloudParams &p = d.params[blk.cloud];
+    if (p.xcross_on) {
+        const double x0 = d.P0[0][g], x1 = d.PS[0][g], xp = p.xcross;
+        if ((x0 < xp && x1 > xp) || (x0 > xp && x1 < xp)) {
+            const int i = atomicAdd(d.xc_count, 1);
+            if (i < d.xc_cap) {
+                XCrossing &c = d.xc[i];
+                c.step = cs.steps;
+                c.cloud = blk.cloud;
+                c.slot = (int)g;
+                c.t = cs.t;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    c.p[k] = d.PS[k][g];
+                    c.v[k] = d.KV[7][k][g];
+                }
+            }
+        }
+    }
+    const bool fsal = p.fsal;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         d.P0[k][g] = d.PS[k][g];

@@ -58,6 +58,9 @@ int sbk_set_step(sbk_engine *, int, double) { return no_device(); }
 int sbk_advance_until(sbk_engine *, double, long long) { return no_device(); }
 int sbk_get_state(sbk_engine *, int, double *, double *) { return no_device(); }
 int sbk_get_status(sbk_engine *, int, sbk_cloud_status *) { return no_device(); }
+int sbk_set_crossing_plane(sbk_engine *, int, int, double, int *) { return no_device(); }
+int sbk_take_crossings(sbk_engine *, int, int *, int *, int *, long long *, double *, double *,
+                       double *) { return no_device(); }
 }
 
 static int failures = 0;

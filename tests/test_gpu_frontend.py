@@ -226,3 +226,75 @@ def test_advance_until_and_t_prev(rng):
         st = e.status(0)
         assert st["t_prev"] < 3.3e-7 <= st["lifetime"]
         assert st["steps"] > 1
+
+
+def _host_crossings(p0, p1, v1, xp, steps, t):
+    """The reference's per-step test (bin/ion_fly.cpp:110-114) on the host."""
+    out = []
+    for i in range(len(p0)):
+        if (p0[i, 0] < xp and p1[i, 0] > xp) or (p0[i, 0] > xp and p1[i, 0] < xp):
+            out.append((steps, i, t, p1[i].copy(), v1[i].copy()))
+    return out
+
+
+def _check_crossings(got, ref):
+    assert len(ref) > 20, len(ref)
+    assert list(got["step"]) == [r[0] for r in ref]
+    assert list(got["ion"]) == [r[1] for r in ref]
+    assert np.array_equal(got["t"], [r[2] for r in ref])
+    assert np.array_equal(got["pos"], np.array([r[3] for r in ref]))
+    assert np.array_equal(got["vel"], np.array([r[4] for r in ref]))
+
+
+def test_crossings_match_a_step_by_step_check(rng):
+    # FP64-exact Penning cloud: the device list after one advance over many
+    # steps against the reference's test applied after each of the same steps
+    amu, m, ch, pos, vel = _rand_cloud(rng, 400)
+    f = S.default_fields(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0)
+    xp = 2e-5
+    with S.Engine(precision="fp64_exact", h0=2e-9) as e:
+        e.add_cloud(pos, vel, m, ch, f)
+        assert e.set_crossing_plane(0, xp) == 65536
+        e.advance(1e-6)
+        got = e.take_crossings()
+        steps = e.status(0)["steps"]
+        assert e.take_crossings()["ion"].size == 0          # taking empties the list
+    ref = []
+    with S.Engine(precision="fp64_exact", h0=2e-9) as e:
+        e.add_cloud(pos, vel, m, ch, f)
+        p0, _ = e.state(0)
+        for _ in range(steps):
+            e.step()
+            p1, v1 = e.state(0)
+            st = e.status(0)
+            ref += _host_crossings(p0, p1, v1, xp, st["steps"], st["lifetime"])
+            p0 = p1
+    _check_crossings(got, ref)
+    assert np.all(got["cloud"] == 0)
+
+
+def test_crossings_survive_reordering_and_can_stop(rng):
+    # mixed precision in spatial order: re-sorting between steps changes what
+    # the device slots mean, so entries still on the list must be mapped to
+    # the caller's ions first
+    amu, m, ch, pos, vel = _rand_cloud(rng, 600)
+    f = S.default_fields(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0)
+    xp = -1e-5
+    with S.Engine(precision="mixed", h0=2e-9, reorder_interval=0) as e:
+        e.add_cloud(pos, vel, m, ch, f)
+        e.set_crossing_plane(0, xp)
+        ref = []
+        p0, _ = e.state(0)
+        for k in range(150):
+            e.step()
+            p1, v1 = e.state(0)
+            st = e.status(0)
+            ref += _host_crossings(p0, p1, v1, xp, st["steps"], st["lifetime"])
+            p0 = p1
+            if k % 7 == 3:
+                e.reorder()
+        _check_crossings(e.take_crossings(), ref)
+        # enable = 0 stops recording
+        e.set_crossing_plane(0, xp, enable=False)
+        e.advance(5e-8)
+        assert e.take_crossings()["ion"].size == 0
