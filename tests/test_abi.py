@@ -93,3 +93,23 @@ def test_cxx_caller_links_against_the_header_only():
     assert os.path.exists(exe), "build tests/cxx (make -C tests/cxx)"
     p = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert p.returncode == 2 and "usage" in p.stderr
+
+
+@pytest.mark.parametrize("call,msg", [
+    ("L.cunbody1_force(None, None, None, ctypes.c_double(0.0), None, -1, 0)",
+     "cunbody1_force_begin: bad arguments"),
+    ("L.cunbody1_force_end(None, None, None, ctypes.c_double(0.0), None, 4, 4)",
+     "no matching cunbody1_force_begin"),
+    ("L.initialize(0, False)", "n_max must be > 0"),
+    ("L.simulation_Async(None, None, 8, 0, 0, 0)", "initialize() was not called"),
+])
+def test_reference_abi_errors_end_the_process(call, msg):
+    # the reference backends' void entry points report errors by printing and
+    # exit(EXIT_FAILURE) (bin/nbody_kernel.cu:37-47); these argument errors are
+    # caught before any device call
+    import subprocess
+    import sys
+    code = f"import ctypes; L = ctypes.CDLL({S.LIB_PATH!r}); {call}; print('returned')"
+    p = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 1, (p.returncode, p.stdout, p.stderr)
+    assert msg in p.stderr and "returned" not in p.stdout
