@@ -305,10 +305,7 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
                 };
                 // sources [j0, j1) of the sub-tile on one path
                 auto run = [&](int j0, int j1, auto far_tag) {
-                    constexpr bool FAR = decltype(far_tag)::value;
-                    // the near path of LOC is the minority: a shorter unroll
-                    // keeps the kernel's register count at the far path's
-                    constexpr int U = (FAR || !LOC) ? 8 : 2;
+                    constexpr int U = 8;
                     if (j1 - j0 == kSubTile) {
 #pragma unroll U
                         for (int j = 0; j < kSubTile; ++j) body(j0 + j, far_tag);
