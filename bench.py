@@ -42,8 +42,8 @@ FLOP_PER_PAIR = 20.0          # GPU Gems 3 n-body convention
 # HBM bytes per launch of the mixed Coulomb kernel at 65,536 ions, from rocprofv3
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB device copy); the
 # automatic split of (4, 4, 0, 1) gives the same 32 slices as (4, 4, 32, 1)
-MEASURED_TRAFFIC = {("gfx950", 65536, (4, 4, 0, 1)): 17.57e6 + 50.33e6,
-                    ("gfx950", 65536, (4, 4, 32, 1)): 17.57e6 + 50.33e6}
+MEASURED_TRAFFIC = {("gfx950", 65536, (4, 4, 0, 1)): 17.65e6 + 50.33e6,
+                    ("gfx950", 65536, (4, 4, 32, 1)): 17.65e6 + 50.33e6}
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector peak
 PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector peak
 HBM_PEAK_GBS = 8000.0
