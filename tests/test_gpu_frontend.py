@@ -298,3 +298,47 @@ def test_crossings_survive_reordering_and_can_stop(rng):
         e.set_crossing_plane(0, xp, enable=False)
         e.advance(5e-8)
         assert e.take_crossings()["ion"].size == 0
+
+
+def test_crossings_of_two_clouds_and_a_short_buffer(rng):
+    # two clouds with their own planes; entries come ordered by cloud, then
+    # step, then ion; moving a plane takes effect from the next step; a
+    # buffer smaller than the list is refused and the entries stay
+    import ctypes
+    clouds = [_rand_cloud(rng, n) for n in (300, 200)]
+    f = S.default_fields(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0)
+    planes = (1e-5, -2e-5)
+    with S.Engine(precision="fp64_exact", h0=2e-9) as e:
+        for amu, m, ch, pos, vel in clouds:
+            e.add_cloud(pos, vel, m, ch, f)
+        for c, xp in enumerate(planes):
+            e.set_crossing_plane(c, xp)
+        ref = {0: [], 1: []}
+        p0 = [e.state(c)[0] for c in (0, 1)]
+        for k in range(120):
+            if k == 60:
+                planes = (planes[0], 3e-5)
+                e.set_crossing_plane(1, planes[1])
+            e.step()
+            for c in (0, 1):
+                p1, v1 = e.state(c)
+                st = e.status(c)
+                ref[c] += _host_crossings(p0[c], p1, v1, planes[c], st["steps"], st["lifetime"])
+                p0[c] = p1
+        n = ctypes.c_int()
+        L = S.lib()
+        assert L.sbk_take_crossings(e._h, 0, ctypes.byref(n), None, None, None, None, None, None) == 0
+        assert n.value == len(ref[0]) + len(ref[1]) > 20
+        ion = np.zeros(n.value, np.int32)
+        rc = L.sbk_take_crossings(e._h, n.value - 1, ctypes.byref(n), None, S._ip(ion), None, None,
+                                  None, None)
+        assert rc == -1                                 # SBK_EINVAL
+        got = e.take_crossings()                        # still all there
+    k0 = len(ref[0])
+    assert list(got["cloud"]) == [0] * k0 + [1] * len(ref[1])
+    sel0 = {key: v[:k0] for key, v in got.items()}
+    sel1 = {key: v[k0:] for key, v in got.items()}
+    for sel, r in ((sel0, ref[0]), (sel1, ref[1])):
+        assert list(sel["step"]) == [x[0] for x in r] and list(sel["ion"]) == [x[1] for x in r]
+        assert np.array_equal(sel["t"], [x[2] for x in r])
+        assert np.array_equal(sel["pos"], np.array([x[3] for x in r]).reshape(-1, 3))
