@@ -284,9 +284,9 @@ int  sbk_eval_coulomb(sbk_engine *e, int cloud, int precision, double *out);
  * {1,4}: wave groups of a 256-thread work-group that share one set of
  * targets and split each source tile; slices = number of source slices per
  * target block (0 = automatic); local_coords (spatial_order mixed engines):
- * 1 lets far source sub-tiles form separations from FP32 offsets, 0 keeps
+ * 1 lets far sources form separations from FP32 offsets, 0 keeps
  * FP64-formed separations everywhere, -1 (automatic) chooses 1 when the
- * clouds' pair-weighted mean size is at least 32768 ions.  Applies to
+ * clouds' pair-weighted mean size is at least 16384 ions.  Applies to
  * SBK_FP64 and SBK_MIXED*; SBK_FP64_EXACT always uses one thread per target
  * and one slice. */
 int  sbk_set_launch(sbk_engine *e, int targets_per_thread, int source_groups, int slices,

@@ -292,11 +292,12 @@ void free_lists(sbk_engine *e)
     destroy_graph(e);
 }
 
-// Whether the mixed Coulomb kernel uses FP32 offsets for far sub-tiles.  It
+// Whether the mixed Coulomb kernel uses FP32 offsets for far sources.  It
 // pays off once a target block is a small part of its cloud (measured on
-// MI355X: 17-26% faster at 65,536-262,144 ions, slower at 10,000 and below,
-// where the per-work-group set-up dominates), hence the automatic rule on
-// the pair-weighted mean cloud size.
+// MI355X with the automatic shape: 13% faster at 20,000 ions, 22-23% at
+// 32,768-65,536, and 23-45% slower at 10,000 and below, where most sources
+// are near and the per-work-group set-up dominates), hence the automatic
+// rule on the pair-weighted mean cloud size.
 int local_coords(const sbk_engine *e, int precision)
 {
     if (!e->spatial || (precision != SBK_MIXED && precision != SBK_MIXED_KAHAN)) return 0;
@@ -306,7 +307,7 @@ int local_coords(const sbk_engine *e, int precision)
         n2 += (double)c.n * c.n;
         n3 += (double)c.n * c.n * c.n;
     }
-    return n2 > 0.0 && n3 / n2 >= 32768.0 ? 1 : 0;
+    return n2 > 0.0 && n3 / n2 >= 16384.0 ? 1 : 0;
 }
 
 // Targets per thread.  0 (the default) picks by size, fitted to the MI355X
