@@ -279,8 +279,9 @@ int  sbk_eval_force(sbk_engine *e, int cloud, double *derivs);
 int  sbk_eval_coulomb(sbk_engine *e, int cloud, int precision, double *out);
 
 /* Coulomb launch shape.  targets_per_thread in {1,2,4,8}, or 0 (the default:
- * 4 when the pair-weighted mean cloud size is >= 32768 ions, else 2 when the
- * clouds hold >= 2.5e8 pairs in total, else 1); source_groups in
+ * 4 when the pair-weighted mean cloud size is >= 16384 ions with local
+ * coordinates or >= 32768 without, else 2 when the clouds hold >= 2.5e8
+ * pairs in total, else 1); source_groups in
  * {1,4}: wave groups of a 256-thread work-group that share one set of
  * targets and split each source tile; slices = number of source slices per
  * target block (0 = automatic); local_coords (spatial_order mixed engines):

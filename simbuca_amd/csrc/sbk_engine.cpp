@@ -310,11 +310,13 @@ int local_coords(const sbk_engine *e, int precision)
     return n2 > 0.0 && n3 / n2 >= 16384.0 ? 1 : 0;
 }
 
-// Targets per thread.  0 (the default) picks by size, fitted to the MI355X
-// tuning table: 4 once a cloud fills the device with long work items (pair-
-// weighted mean size >= 32768), 2 for ensembles with many pairs in total,
-// 1 for small clouds, where more, shorter work-groups win.
-int targets_per_thread(const sbk_engine *e)
+// Targets per thread.  0 (the default) picks by size, fitted to MI355X
+// measurements: 4 once a cloud fills the device with long work items (pair-
+// weighted mean size >= 16384 with local coordinates, whose far path reuses
+// each staged source over 4 targets best: 16,384-32,768 ions 2-6% faster
+// than 2; >= 32768 without), 2 for ensembles with many pairs in total, 1
+// for small clouds, where more, shorter work-groups win.
+int targets_per_thread(const sbk_engine *e, int precision)
 {
     if (e->tpt > 0) return e->tpt;
     double n2 = 0.0, n3 = 0.0;
@@ -322,7 +324,8 @@ int targets_per_thread(const sbk_engine *e)
         n2 += (double)c.n * c.n;
         n3 += (double)c.n * c.n * c.n;
     }
-    if (n2 > 0.0 && n3 / n2 >= 32768.0) return 4;
+    const double big = local_coords(e, precision) ? 16384.0 : 32768.0;
+    if (n2 > 0.0 && n3 / n2 >= big) return 4;
     return n2 >= 2.5e8 ? 2 : 1;
 }
 
@@ -331,7 +334,7 @@ int targets_per_thread(const sbk_engine *e)
 int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
 {
     const bool exact = precision == SBK_FP64_EXACT;
-    const int tpt = exact ? 1 : targets_per_thread(e);
+    const int tpt = exact ? 1 : targets_per_thread(e, precision);
     const int q = exact ? 1 : e->groups;
     const int sreq = exact ? 1 : e->slices_req;
     auto key = std::make_tuple(precision, tpt, q, sreq, only_coulomb ? 1 : 0,
@@ -734,7 +737,7 @@ int launch_coulomb_timed(sbk_engine *e, int precision, const sbk::CoulombArgs &a
     sbk::CoulombArgs b = a;
     if (precision == SBK_FP64) b.eps_d = 1e-40;   // the guard of bin/force.cpp:136
     b.local_coords = local_coords(e, precision);
-    if (sbk::launch_coulomb(kp(precision), targets_per_thread(e), e->groups, b, e->stream))
+    if (sbk::launch_coulomb(kp(precision), targets_per_thread(e, precision), e->groups, b, e->stream))
         return fail(SBK_EHIP, "Coulomb kernel launch failed: %s",
                     hipGetErrorString(hipGetLastError()));
     if (evp) HIP_TRY(hipEventRecord(evp->second, e->stream));
@@ -1431,7 +1434,7 @@ static int time_coulomb_pass(sbk_engine *e, int precision, int reps, bool with_s
     HIP_TRY(hipEventCreate(&t1));
     HIP_TRY(hipEventRecord(t0, e->stream));
     for (int i = 0; i < reps; ++i) {
-        if (sbk::launch_coulomb(kp(precision), targets_per_thread(e), e->groups, a, e->stream))
+        if (sbk::launch_coulomb(kp(precision), targets_per_thread(e, precision), e->groups, a, e->stream))
             return fail(SBK_EHIP, "Coulomb kernel launch failed");
         if (with_sum && sbk::launch_eval_derivs(d, 0, e->stream))
             return fail(SBK_EHIP, "partial-sum kernel launch failed");
