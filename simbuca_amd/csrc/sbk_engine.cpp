@@ -1481,6 +1481,16 @@ int sbk_autotune(sbk_engine *e, int precision, int *tpt_out, int *groups_out, in
     sbk_device_info(dev_name, sizeof(dev_name), nullptr, nullptr);
     FILE *fp = table ? std::fopen(table, "a") : nullptr;
     const int old_t = e->tpt, old_q = e->groups, old_s = e->slices_req, old_l = e->local_req;
+    {
+        // about 50 ms of launches first: the first shape timed on an idle
+        // device otherwise reads several per cent slow while its clocks rise
+        double ms0 = 0.0;
+        rc = time_coulomb_pass(e, precision, 1, true, &ms0);
+        if (!rc) {
+            const int warm = (int)std::min(200.0, std::max(1.0, 50.0 / std::max(ms0, 1e-3)));
+            rc = time_coulomb_pass(e, precision, warm, true, &ms0);
+        }
+    }
     for (int l = 0; l <= (may_local ? 1 : 0) && !rc; ++l) {
         e->local_req = l;
         for (int t : tpts) {
