@@ -218,15 +218,19 @@ int kp(int precision)
 // the ions of a block are close together.  A leaf (at most 64 ions) is
 // sorted by position, so the order depends only on the positions, not on
 // the order they came in: re-sorting a sorted cloud changes nothing.
-// pos is AoS [n][3].
+// pos is AoS [n][3].  A NaN coordinate sorts as +infinity, so the
+// comparisons stay a strict weak order whatever the positions hold.
+inline double order_key(double v) { return std::isnan(v) ? INFINITY : v; }
+
 void bisect(const double *pos, int *idx, int len)
 {
     if (len <= 64) {
         std::sort(idx, idx + len, [pos](int a, int b) {
             const double *p = pos + 3 * (size_t)a, *q = pos + 3 * (size_t)b;
-            if (p[0] != q[0]) return p[0] < q[0];
-            if (p[1] != q[1]) return p[1] < q[1];
-            if (p[2] != q[2]) return p[2] < q[2];
+            for (int k = 0; k < 3; ++k) {
+                const double u = order_key(p[k]), v = order_key(q[k]);
+                if (u != v) return u < v;
+            }
             return a < b;
         });
         return;
@@ -245,7 +249,7 @@ void bisect(const double *pos, int *idx, int len)
     while (A > 64 && 2 * A > len) A /= 2;
     const int m = std::max(A, (int)std::lround(len / (2.0 * A)) * A);   // A <= m < len
     std::nth_element(idx, idx + m, idx + len, [pos, ax](int a, int b) {
-        return pos[3 * (size_t)a + ax] < pos[3 * (size_t)b + ax];
+        return order_key(pos[3 * (size_t)a + ax]) < order_key(pos[3 * (size_t)b + ax]);
     });
     bisect(pos, idx, m);
     bisect(pos, idx + m, len - m);
@@ -1042,6 +1046,9 @@ int sbk_add_cloud(sbk_engine *e, int n, const double *pos, const double *vel, co
     c.charge.assign(charge, charge + n);
     for (int i = 0; i < n; ++i)
         if (!(c.mass[i] > 0.0)) return fail(SBK_EINVAL, "ion %d has mass <= 0", i);
+    for (size_t i = 0; i < 3 * (size_t)n; ++i)
+        if (!std::isfinite(c.pos[i]) || !std::isfinite(c.vel[i]))
+            return fail(SBK_EINVAL, "ion %zu has a non-finite position or velocity", i / 3);
     if (f->swift_n > 0) c.swift.assign(f->swift, f->swift + f->swift_n);
     c.f.swift = nullptr;   // the samples live in c.swift from here on
     e->clouds.push_back(std::move(c));

@@ -794,3 +794,14 @@ def test_nbody_abi(rng, n):
         return
     err = np.linalg.norm(acc - ref) / np.linalg.norm(ref)
     assert err < 1e-6, err
+
+
+def test_add_cloud_rejects_non_finite_state(rng):
+    amu, m, ch, pos, vel = _rand_cloud(rng, 10)
+    sf, _ = _both_fields(trap=0, coulomb=1)
+    for bad_pos in (True, False):
+        p, v = pos.copy(), vel.copy()
+        (p if bad_pos else v)[3, 1] = np.nan if bad_pos else np.inf
+        with S.Engine(precision="mixed") as e:
+            with pytest.raises(S.SimbucaError, match="non-finite"):
+                e.add_cloud(p, v, m, ch, sf)

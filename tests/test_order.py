@@ -57,3 +57,16 @@ def test_leaf_boundaries_are_aligned():
     for b in range(0, 5000 - 64 + 1, 64):
         x = q[b:b + 64, 0]
         assert np.all(np.diff(x) >= 0)
+
+
+@pytest.mark.parametrize("n", [50, 3000])
+def test_order_with_non_finite_positions_is_a_permutation(n):
+    # NaN sorts as +infinity, so the sort and the partition keep a strict
+    # weak order: the result is still a permutation of the ions
+    rng = np.random.default_rng(7)
+    pos = rng.normal(0, 1e-3, (n, 3))
+    bad = rng.choice(n, max(1, n // 20), replace=False)
+    pos[bad, rng.integers(0, 3, bad.size)] = np.nan
+    pos[bad[:1], 0] = np.inf
+    order = S.spatial_order(pos)
+    assert np.array_equal(np.sort(order), np.arange(n))
