@@ -544,6 +544,9 @@ __device__ __forceinline__ void coulomb_total(const EngineDev &d, const CloudPar
     Fx = d.CP[g];
     Fy = d.CP[st + g];
     Fz = d.CP[2 * st + g];
+    // unrolled so that the loads of several slices are in flight at once;
+    // the sum keeps its slice order
+#pragma unroll 8
     for (int s = 1; s < ns; ++s) {
         const double *cp = d.CP + (long long)s * 3 * st + g;
         Fx += cp[0];
