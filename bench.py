@@ -212,34 +212,13 @@ def force_error(S, w, e_mixed):
     }
 
 
-def cloud_invariants(c, pos, vel, B, cpen):
-    """Total energy (kinetic + trap + Coulomb) and canonical angular momentum
-    L_z = sum m (x v_y - y v_x) + q e B (x^2 + y^2)/2 of one cloud."""
-    from simbuca_amd import EL_CHARGE, KE
-    q = c.charge.astype(np.float64) * EL_CHARGE
-    kin = 0.5 * (c.mass[:, None] * vel * vel).sum()
-    trap = (q * cpen * (pos[:, 2] ** 2 - 0.5 * (pos[:, 0] ** 2 + pos[:, 1] ** 2))).sum()
-    coul = 0.0
-    for i in range(0, c.n, 512):
-        d = pos[i:i + 512, None, :] - pos[None, :, :]
-        r = np.sqrt((d * d).sum(-1))
-        idx = np.arange(i, min(i + 512, c.n))
-        r[idx - i, idx] = np.inf
-        coul += 0.5 * KE * (q[i:i + 512, None] * q[None, :] / r).sum()
-    lz = (c.mass * (pos[:, 0] * vel[:, 1] - pos[:, 1] * vel[:, 0])).sum() + \
-        (q * B * (pos[:, 0] ** 2 + pos[:, 1] ** 2) / 2).sum()
-    return kin + trap + coul, lz
-
-
 def invariant_drift(S):
     """Energy and canonical L_z drift over the whole 100 us of the 500-ion
     conservative Penning cloud (B, quadrupole, Coulomb; no drive, no friction)."""
     from simbuca_amd import synth
     w = synth.workload(1)
     c = w.clouds[0]
-    B, V0, d0 = c.fields.Bz, c.fields.V0, c.fields.d0
-    cpen = V0 / (2 * d0 * d0)
-    e0, l0 = cloud_invariants(c, c.pos, c.vel, B, cpen)
+    e0, l0 = synth.cloud_invariants(c, c.pos, c.vel)
     out = {"workload": w.name, "t_sim": w.t_sim}
     for p in ("mixed", "fp64"):
         e = make_engine(S, w, p, timing=False)
@@ -247,7 +226,7 @@ def invariant_drift(S):
         pos, vel = e.state(0)
         st = e.status(0)
         e.close()
-        e1, l1 = cloud_invariants(c, pos, vel, B, cpen)
+        e1, l1 = synth.cloud_invariants(c, pos, vel)
         out[p] = {"steps": st["steps"], "rel_energy_drift": (e1 - e0) / abs(e0),
                   "rel_Lz_drift": (l1 - l0) / abs(l0)}
     return out

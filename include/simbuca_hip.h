@@ -181,6 +181,10 @@ typedef struct sbk_config {
                                 in the caller's order either way */
     int    reorder_interval; /* attempts between re-sorts of that order as the
                                 ions move (0: sort once, at the first run) */
+    double beta;        /* PI control of the step controller: 0 (the reference's
+                           value, default) or e.g. 0.04..0.08, the reference's
+                           documented switch (bin/ode.cpp:427-432); alpha =
+                           0.2 - 0.75 beta.  Must be in [0, 0.2). */
 } sbk_config;
 
 typedef struct sbk_cloud_status {

@@ -124,6 +124,10 @@ def lib():
                                         ctypes.c_int]
         L.orc_controller.restype = ctypes.c_int
         L.orc_controller.argtypes = [ctypes.c_double, dp, dp, dp, ip]
+        L.orc_controller_beta.restype = ctypes.c_int
+        L.orc_controller_beta.argtypes = [ctypes.c_double, ctypes.c_double, dp, dp, dp, ip]
+        L.orc_set_beta.restype = None
+        L.orc_set_beta.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.orc_create.restype = ctypes.c_void_p
         L.orc_create.argtypes = [ctypes.c_int, dp, dp, dp, ip, ctypes.POINTER(Fields),
                                  ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -199,12 +203,13 @@ def cunbody_force(xj, mj, xi, eps2):
     return out
 
 
-def controller(err, h, hnext, errold, reject):
-    """error_succes on a scalar error: returns (accept, h, hnext, errold, reject)."""
+def controller(err, h, hnext, errold, reject, beta=0.0):
+    """error_succes on a scalar error: returns (accept, h, hnext, errold, reject).
+    beta != 0 is the reference's documented PI switch (bin/ode.cpp:427-432)."""
     H = ctypes.c_double(h); Hn = ctypes.c_double(hnext); Eo = ctypes.c_double(errold)
     R = ctypes.c_int(int(reject))
-    ok = lib().orc_controller(float(err), ctypes.byref(H), ctypes.byref(Hn),
-                              ctypes.byref(Eo), ctypes.byref(R))
+    ok = lib().orc_controller_beta(float(err), float(beta), ctypes.byref(H), ctypes.byref(Hn),
+                                   ctypes.byref(Eo), ctypes.byref(R))
     return bool(ok), H.value, Hn.value, Eo.value, bool(R.value)
 
 
@@ -212,7 +217,7 @@ class Cloud:
     """Restated IonCloud + _ode_vars pair driven by step()/move_particles()."""
 
     def __init__(self, pos, vel, mass, charge, f: Fields, lifetime=0.0, h0=1e-9,
-                 rtol=1e-7, atol=1e-7, adaptive=True, time_mode=TIME_FROZEN):
+                 rtol=1e-7, atol=1e-7, adaptive=True, time_mode=TIME_FROZEN, beta=0.0):
         pos = _f64(pos, (-1, 3)); vel = _f64(vel, (-1, 3))
         mass = _f64(mass); charge = _i32(charge)
         self.n = pos.shape[0]
@@ -220,6 +225,8 @@ class Cloud:
                                    ctypes.byref(f), lifetime, h0, rtol, atol,
                                    1 if adaptive else 0, time_mode)
         self._fields = f       # the C copy points at its SWIFT samples
+        if beta:
+            lib().orc_set_beta(self._h, float(beta))
 
     def __del__(self):
         if getattr(self, "_h", None):

@@ -49,7 +49,7 @@ struct orc_cloud {
     double (*derivs)[3];
     double h, hnext, errold, last_err;
     int reject;
-    double rtol, atol;
+    double rtol, atol, beta;
     int adaptive, time_mode;
     orc_fields f;
     double ramp_Vrf0, ramp_Vdc0;
@@ -247,16 +247,21 @@ void orc_coulomb_rows(int n, const double *pos_, const double *mass, const int *
 }
 
 /* Interface semantics of cunbody1_force (bin/cunbody.h:11-33, as called at
- * bin/cunbody.cpp:65-66,88): softened direct sum with source weights mj. */
+ * bin/cunbody.cpp:65-66,88): softened direct sum with source weights mj.
+ * The reference passes the same positions as xj and, copied, as xi; then
+ * the pair j == i is skipped (its numerator is 0, so with eps2 > 0 it adds
+ * nothing, and with eps2 == 0 it would be 0/0). */
 void orc_cunbody_force(const double *xj_, const double *mj, const double *xi_, double eps2,
                        double *ai_, int ni, int nj)
 {
     const double (*xj)[3] = (const double (*)[3])xj_;
     const double (*xi)[3] = (const double (*)[3])xi_;
     double (*ai)[3] = (double (*)[3])ai_;
+    const int alias = ni == nj && (xj_ == xi_ || memcmp(xj_, xi_, sizeof(double) * 3 * (size_t)ni) == 0);
     for (int i = 0; i < ni; i++) {
         double ax = 0.0, ay = 0.0, az = 0.0;
         for (int j = 0; j < nj; j++) {
+            if (alias && j == i) continue;
             double rx = xj[j][0] - xi[i][0];
             double ry = xj[j][1] - xi[i][1];
             double rz = xj[j][2] - xi[i][2];
@@ -273,11 +278,13 @@ void orc_cunbody_force(const double *xj_, const double *mj, const double *xi_, d
     }
 }
 
-/* bin/ode.cpp:426-457 (Numerical Recipes 3rd ed. Controller::success, beta=0) */
-int orc_controller(double err, double *h, double *hnext, double *errold, int *reject)
+/* bin/ode.cpp:426-457 (Numerical Recipes 3rd ed. Controller::success).
+ * The reference fixes beta = 0 and documents beta = 0.04..0.08 as the switch
+ * to PI control (bin/ode.cpp:427-432); alpha = 0.2 - 0.75 beta follows it. */
+int orc_controller_beta(double err, double beta, double *h, double *hnext, double *errold,
+                        int *reject)
 {
-    static const double beta = 0.0, alpha = 0.2 - beta * 0.75, safe = 0.9, minscale = 0.2,
-                        maxscale = 10.0;
+    const double alpha = 0.2 - beta * 0.75, safe = 0.9, minscale = 0.2, maxscale = 10.0;
     double scale;
     if (err <= 1.0) {
         if (err == 0.0)
@@ -299,6 +306,16 @@ int orc_controller(double err, double *h, double *hnext, double *errold, int *re
     *h *= scale;
     *reject = 1;
     return 0;
+}
+
+int orc_controller(double err, double *h, double *hnext, double *errold, int *reject)
+{
+    return orc_controller_beta(err, 0.0, h, hnext, errold, reject);
+}
+
+void orc_set_beta(orc_cloud *c, double beta)
+{
+    c->beta = beta;
 }
 
 static void *xalloc(size_t bytes)
@@ -481,7 +498,7 @@ int orc_step(orc_cloud *c)
     for (;;) {
         double err = orc_dp5_attempt(c);
         if (c->adaptive) {
-            if (orc_controller(err, &c->h, &c->hnext, &c->errold, &c->reject)) break;
+            if (orc_controller_beta(err, c->beta, &c->h, &c->hnext, &c->errold, &c->reject)) break;
             if (fabs(c->h) <= fabs(c->pos[0][0]) * DBL_EPSILON) return -1;
         } else
             break;

@@ -93,11 +93,16 @@ void orc_cunbody_force(const double *xj, const double *mj, const double *xi, dou
 
 /* Controller of ode.cpp:426-457 on a scalar error. Returns 1 on accept. */
 int orc_controller(double err, double *h, double *hnext, double *errold, int *reject);
+/* the same with PI control: beta = 0 is orc_controller (bin/ode.cpp:427-432) */
+int orc_controller_beta(double err, double beta, double *h, double *hnext, double *errold,
+                        int *reject);
 
 orc_cloud *orc_create(int n, const double *pos, const double *vel, const double *mass,
                       const int *charge, const orc_fields *f, double lifetime, double h0,
                       double rtol, double atol, int adaptive, int time_mode);
 void   orc_destroy(orc_cloud *c);
+/* PI control of the cloud's step controller (0 at creation) */
+void   orc_set_beta(orc_cloud *c, double beta);
 void   orc_set_fields(orc_cloud *c, const orc_fields *f);
 /* One Dormand_Prince_5 attempt from (pos, vel); leaves y_{n+1} in pos2/vel2. */
 double orc_dp5_attempt(orc_cloud *c);

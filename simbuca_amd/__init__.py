@@ -83,6 +83,7 @@ class Config(ctypes.Structure):
         ("use_graph", ctypes.c_int), ("timing", ctypes.c_int),
         ("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("h0", ctypes.c_double),
         ("spatial_order", ctypes.c_int), ("reorder_interval", ctypes.c_int),
+        ("beta", ctypes.c_double),
     ]
 
 
@@ -421,7 +422,7 @@ class Engine:
 
     def __init__(self, precision="mixed", time_mode=TIME_FROZEN, adaptive=True, use_graph=True,
                  timing=False, rtol=1e-7, atol=1e-7, h0=1e-9, spatial_order=None,
-                 reorder_interval=None):
+                 reorder_interval=None, beta=0.0):
         L = lib()
         cfg = Config()
         L.sbk_default_config(ctypes.byref(cfg))
@@ -435,6 +436,7 @@ class Engine:
             cfg.spatial_order = 1 if spatial_order else 0
         if reorder_interval is not None:
             cfg.reorder_interval = int(reorder_interval)
+        cfg.beta = float(beta)
         h = ctypes.c_void_p()
         _check(L.sbk_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

@@ -237,10 +237,15 @@ void cunbody1_force_begin(double xj[][3], double mj[], double xi[][3], double ep
     a.stride = stride;
     a.items = g.d_items;
     a.nitems = g.nitems;
-    a.self_skip = 0;
+    // The reference passes one position set as xj and, copied, as xi
+    // (bin/cunbody.cpp:65-66): then each ion meets itself.  The FP64-exact sum
+    // skips that pair by index; the tiled sums keep it finite with a floor on
+    // the softening (the self pair then adds exactly 0), so eps2 = 0 works.
+    const bool alias = ni == nj && (xi == xj || std::memcmp(xi, xj, sizeof(double) * 3 * (size_t)ni) == 0);
+    a.self_skip = alias ? 1 : 0;
     a.unit_weights = unit;
     a.state = nullptr;
-    a.eps_d = eps2;
+    a.eps_d = exact ? eps2 : std::max(eps2, 1e-40);   // 1e-40: the guard of bin/force.cpp:136
     a.eps_f = std::max((float)eps2, 1e-23f);   // FP32 needs eps >= 1e-23 to keep r=0 finite
     if (sbk::launch_coulomb(g.precision, tpt, q, a, g.stream)) die("Coulomb kernel launch", hipGetLastError());
     hipLaunchKernelGGL(reduce_slices, dim3((ni + 255) / 256), dim3(256), 0, g.stream,

@@ -24,6 +24,7 @@
 
 #include "../../include/simbuca_hip.h"
 #include "sbk_internal.h"
+#include "sbk_plan.h"
 
 namespace {
 
@@ -278,8 +279,12 @@ inline int caller_index(const HostCloud &c, int i)
     return c.perm.empty() ? i : c.perm[i];
 }
 
+// The attempt graph's kernels hold the work list, partial-sum buffer and
+// ion arrays by address: the stream is drained first, so no replay still in
+// flight reads memory that the caller frees or reallocates next.
 void destroy_graph(sbk_engine *e)
 {
+    if ((e->graph_exec || e->graph) && e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
     e->graph_exec = nullptr;
@@ -288,12 +293,13 @@ void destroy_graph(sbk_engine *e)
 
 void free_lists(sbk_engine *e)
 {
+    destroy_graph(e);   // first: the graph reads the lists
+    if (e->stream) (void)hipStreamSynchronize(e->stream);   // eager launches read them too
     for (auto &kv : e->lists) {
         (void)hipFree(kv.second.d_items);
         (void)hipFree(kv.second.d_slices);
     }
     e->lists.clear();
-    destroy_graph(e);
 }
 
 // Whether the mixed Coulomb kernel uses FP32 offsets for far sources.  It
@@ -349,75 +355,23 @@ int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
         return 0;
     }
     const int blk = sbk::kBlock / q * tpt;   // targets per work-group
-    long long total_blocks = 0;
-    for (auto &c : e->clouds)
-        if (c.n > 0 && (!only_coulomb || c.f.coulomb)) total_blocks += (c.n + blk - 1) / blk;
-    // automatic split: about eight rounds of resident work-groups; short
-    // items keep the last, partially filled round (the tail) short
     const long long capacity =
         (long long)sbk::coulomb_blocks_per_cu(precision, tpt, e->unit_charges ? 1 : 0, q,
                                               local_coords(e, precision)) *
         e->cus;
-    const long long target = 8 * capacity;
-    ItemList L;
-    std::vector<sbk::CoulombItem> items;
-    L.slices.assign(e->clouds.size(), 1);
+    std::vector<sbk::PlanCloud> pc(e->clouds.size());
     for (size_t ci = 0; ci < e->clouds.size(); ++ci) {
-        const HostCloud &c = e->clouds[ci];
-        if (c.n <= 0 || (only_coulomb && !c.f.coulomb)) continue;
-        const int ib = (c.n + blk - 1) / blk;
-        int S = 1;
-        if (precision != SBK_FP64_EXACT) {
-            if (sreq > 0)
-                S = sreq;
-            else {
-                long long want = total_blocks > 0 ? (target + total_blocks / 2) / total_blocks : 1;
-                // at least 512 sources per slice (partial sums stay a few % of the traffic),
-                // relaxed to 64 for clouds too small to fill the device otherwise
-                long long cap = std::max(1, c.n / (total_blocks * 8 < capacity ? 64 : 512));
-                S = (int)std::max(1LL, std::min(want, cap));
-            }
-        }
-        S = std::max(1, std::min(S, c.n));
-        // slice length: a multiple of the sources one LDS tile gives all Q
-        // wave groups (each group takes its own kSubTile of every tile), so
-        // that only a cloud's last slice ends in a partial tile
-        const int quantum = sbk::kSubTile * q;
-        int chunk = (c.n + S - 1) / S;
-        chunk = ((chunk + quantum - 1) / quantum) * quantum;
-        S = (c.n + chunk - 1) / chunk;
-        L.slices[ci] = S;
-        L.max_slices = std::max(L.max_slices, S);
-        for (int s = 0; s < S; ++s) {
-            const int j0 = c.off + s * chunk, j1 = c.off + std::min(c.n, (s + 1) * chunk);
-            for (int b = 0; b < ib; ++b) {
-                sbk::CoulombItem w{};
-                w.cloud = (int)ci;
-                w.i0 = c.off + b * blk;
-                w.i1 = c.off + std::min(c.n, (b + 1) * blk);
-                w.j0 = j0;
-                w.j1 = j1;
-                w.slice = s;
-                items.push_back(w);
-            }
-        }
+        pc[ci].n = e->clouds[ci].n;
+        pc[ci].off = e->clouds[ci].off;
+        pc[ci].on = !only_coulomb || e->clouds[ci].f.coulomb;
     }
-    // Work-group b runs on XCD b % 8 and takes the b / 8-th item of that
-    // XCD's contiguous run (sbk::xcd_item); an XCD starts its work-groups
-    // in order.  Each run is therefore ordered costliest first (an item
-    // costs its source count: every lane loops over all of them), so that
-    // short items fill the last, partly empty round of the XCD; the runs
-    // themselves keep their clouds, which keeps each XCD's sources in its L2.
-    {
-        const int nb = (int)items.size(), qx = nb / 8, rx = nb % 8;
-        for (int x = 0; x < 8; ++x) {
-            const int b0 = x * qx + std::min(x, rx), cnt = qx + (x < rx ? 1 : 0);
-            std::stable_sort(items.begin() + b0, items.begin() + b0 + cnt,
-                             [](const sbk::CoulombItem &u, const sbk::CoulombItem &v) {
-                                 return u.j1 - u.j0 > v.j1 - v.j0;
-                             });
-        }
-    }
+    sbk::Plan plan = sbk::plan_coulomb(pc, blk, sbk::kSubTile * q, sreq, capacity, exact);
+    const std::string bad = sbk::check_plan(pc, plan);
+    if (!bad.empty()) return fail(SBK_ESTATE, "Coulomb work list is inconsistent: %s", bad.c_str());
+    ItemList L;
+    std::vector<sbk::CoulombItem> &items = plan.items;
+    L.slices = plan.slices;
+    L.max_slices = plan.max_slices;
     L.nitems = (int)items.size();
     if (L.nitems > 0) {
         HIP_TRY(hipMalloc(&L.d_items, sizeof(sbk::CoulombItem) * items.size()));
@@ -429,6 +383,7 @@ int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
                       hipMemcpyHostToDevice));
     if (L.max_slices > e->cp_slices) {
         destroy_graph(e);
+        HIP_TRY(hipStreamSynchronize(e->stream));   // launches in flight may still write d_cp
         if (e->d_cp) HIP_TRY(hipFree(e->d_cp));
         e->d_cp = nullptr;
         HIP_TRY(hipMalloc(&e->d_cp, sizeof(double) * 3 * e->stride * L.max_slices));
@@ -437,6 +392,16 @@ int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
     }
     auto res = e->lists.emplace(key, L);
     *out = &res.first->second;
+    return 0;
+}
+
+// The partial-sum buffer holds cp_slices planes; a list whose items write
+// more planes must never be launched (get_list grows the buffer first).
+int check_partials(const sbk_engine *e, const ItemList *L)
+{
+    if (L->max_slices > e->cp_slices || (L->nitems > 0 && !e->d_cp))
+        return fail(SBK_ESTATE, "partial-sum buffer holds %d slices, the work list writes %d",
+                    e->cp_slices, L->max_slices);
     return 0;
 }
 
@@ -457,6 +422,9 @@ sbk::CloudParams cloud_params(const sbk_engine *e, size_t ci)
     p.fsal = !c.f.ramp && (e->cfg.time_mode == SBK_TIME_STAGE || !timed);
     p.xcross_on = c.xcross_on;
     p.xcross = c.xcross;
+    p.ion0 = c.off;
+    for (size_t i = 0; i < c.perm.size(); ++i)
+        if (c.perm[i] == 0) p.ion0 = c.off + (int)i;
     return p;
 }
 
@@ -687,6 +655,7 @@ int finalize(sbk_engine *e)
     d.adaptive = e->cfg.adaptive;
     d.rtol = e->cfg.rtol;
     d.atol = e->cfg.atol;
+    d.beta = e->cfg.beta;
     if (int rc2 = ensure_crossings(e)) return rc2;
     d.CP = nullptr;
     d.slices = nullptr;
@@ -768,6 +737,7 @@ int enqueue_attempt(sbk_engine *e)
     for (auto &c : e->clouds) any_coulomb |= (c.f.coulomb && c.n > 0);
     int rc = get_list(e, e->cfg.precision, true, &L);
     if (rc) return rc;
+    if ((rc = check_partials(e, L))) return rc;
     sbk::EngineDev d = e->dev;
     d.slices = L->d_slices;
     for (int s = 1; s <= 7; ++s) {
@@ -871,7 +841,7 @@ int reorder(sbk_engine *e)
     HIP_TRY(hipMemcpyAsync(e->d_qf, tmpf, sizeof(float) * n, hipMemcpyDeviceToDevice, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));   // src goes out of scope
     e->reorders++;
-    return 0;
+    return upload_params(e);   // the slot of each cloud's ion 0 moved
 }
 
 // Re-sorts when reorder_interval attempts have run since the last sort; the
@@ -971,6 +941,8 @@ int sbk_create(const sbk_config *cfg, sbk_engine **out)
     if (!(cfg->h0 > 0.0) || !(cfg->rtol >= 0.0) || !(cfg->atol >= 0.0) ||
         (cfg->rtol == 0.0 && cfg->atol == 0.0))
         return fail(SBK_EINVAL, "h0 must be > 0 and rtol/atol >= 0, not both 0");
+    if (!(cfg->beta >= 0.0 && cfg->beta < 0.2))
+        return fail(SBK_EINVAL, "beta must be in [0, 0.2) (got %g)", cfg->beta);
     std::unique_ptr<sbk_engine> e(new sbk_engine);
     e->cfg = *cfg;
     HIP_TRY(hipGetDevice(&e->device));
@@ -1371,6 +1343,7 @@ static int eval_common(sbk_engine *e, int cloud, int precision, int with_fields,
     ItemList *L = nullptr;
     rc = get_list(e, precision, false, &L);
     if (rc) return rc;
+    if ((rc = check_partials(e, L))) return rc;
     sbk::EngineDev d = e->dev;
     d.slices = L->d_slices;
     const bool need = !with_fields || c.f.coulomb;
@@ -1431,6 +1404,7 @@ static int time_coulomb_pass(sbk_engine *e, int precision, int reps, bool with_s
     ItemList *L = nullptr;
     int rc = get_list(e, precision, false, &L);
     if (rc) return rc;
+    if ((rc = check_partials(e, L))) return rc;
     sbk::CoulombArgs a = coulomb_args(e, *L, e->dev.P0, false);
     if (precision == SBK_FP64) a.eps_d = 1e-40;
     a.local_coords = local_coords(e, precision);

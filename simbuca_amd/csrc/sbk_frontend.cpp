@@ -988,7 +988,8 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
         log.line("Normal trap operation for %g sec,\n", op.time);   // normal_operation, bin/ion_fly.cpp:139-143
         if (verbose)
             std::fprintf(stderr, "operation %s for %g s from t = %g s\n", op.name.c_str(), op.time, t_start);
-        while (st.lifetime < endtime) {
+        // move_particles leaves at once when the cloud is empty (bin/ion_fly.cpp:87-88)
+        while (N > 0 && st.lifetime < endtime) {
             const double t_print = print_interval * nr_interval;
             double target = st.lifetime;
             if (!(target - P.lifetime0 > t_print)) {
@@ -1002,8 +1003,13 @@ extern "C" int sbk_run_ini(const char *ini_path, const sbk_mass_table *table, in
             if (target > endtime) target = endtime;
             const long long max_attempts =
                 P.print_at_x ? std::max(1, xc_cap / std::max(N, 1)) : (1LL << 62);
+            const double before = st.lifetime;
             if (int rc = sbk_advance_until(e, target, max_attempts)) return rc;
             if (int rc = sbk_get_status(e, 0, &st)) return rc;
+            // an advance that stopped with the cloud idle, no error and no time gained
+            // would repeat forever
+            if (st.lifetime == before && !st.active && !st.error)
+                return ffail(SBK_ESTATE, "the engine made no progress at lifetime %.17g s", before);
             if (P.print_at_x)
                 if (int rc = write_crossings()) return rc;
             const bool due = st.lifetime - P.lifetime0 > t_print;

@@ -55,6 +55,8 @@ struct CloudParams {
     int fsal;                 // an accepted attempt's stage-7 derivative is exactly the
                               // next step's stage-1 derivative (same state and time)
     int xcross_on;            // record the ions whose x crosses xcross in an accepted step
+    int ion0;                 // slot (global index) of the caller's ion 0: its x scales the
+                              // step-size underflow test (bin/ode.cpp:482)
     double xcross;
 };
 
@@ -124,6 +126,7 @@ struct EngineDev {
     int ntot, nclouds;
     int time_mode, adaptive;
     double rtol, atol;
+    double beta;                  // PI control of the step controller (0: the reference's)
     XCrossing *xc;                // crossing list (nullptr: no cloud records crossings)
     int *xc_count;                // entries written since the list was last emptied
     int xc_cap;                   // entries the list holds

@@ -667,7 +667,7 @@ __global__ __launch_bounds__(256) void controller_kernel(EngineDev d)
     cs.evals += cs.k1_valid ? 6 : 7;
     int accept = 1;
     if (d.adaptive) {
-        const double beta = 0.0, alpha = 0.2 - beta * 0.75, safe = 0.9, minscale = 0.2,
+        const double beta = d.beta, alpha = 0.2 - beta * 0.75, safe = 0.9, minscale = 0.2,
                      maxscale = 10.0;
         double scale;
         if (err <= 1.0) {
@@ -689,7 +689,7 @@ __global__ __launch_bounds__(256) void controller_kernel(EngineDev d)
             cs.h *= scale;
             cs.reject = 1;
             accept = 0;
-            if (fabs(cs.h) <= fabs(d.P0[0][p.off]) * DBL_EPSILON) {
+            if (fabs(cs.h) <= fabs(d.P0[0][p.ion0]) * DBL_EPSILON) {
                 cs.error = 1;
                 cs.active = 0;
             }

@@ -1,0 +1,142 @@
+// sbk_plan.cpp -- the Coulomb work list of an engine's clouds (host only).
+#include "sbk_plan.h"
+
+#include <algorithm>
+#include <cstdio>
+
+namespace sbk {
+
+Plan plan_coulomb(const std::vector<PlanCloud> &clouds, int blk, int quantum, int sreq,
+                  long long capacity, bool exact)
+{
+    Plan L;
+    L.slices.assign(clouds.size(), 1);
+    long long total_blocks = 0;
+    for (const PlanCloud &c : clouds)
+        if (c.n > 0 && c.on) total_blocks += (c.n + blk - 1) / blk;
+    // automatic split: about eight rounds of resident work-groups; short
+    // items keep the last, partially filled round (the tail) short
+    const long long target = 8 * capacity;
+    for (size_t ci = 0; ci < clouds.size(); ++ci) {
+        const PlanCloud &c = clouds[ci];
+        if (c.n <= 0 || !c.on) continue;
+        const int ib = (c.n + blk - 1) / blk;
+        int S = 1;
+        if (!exact) {
+            if (sreq > 0)
+                S = sreq;
+            else {
+                long long want = total_blocks > 0 ? (target + total_blocks / 2) / total_blocks : 1;
+                // at least 512 sources per slice (partial sums stay a few % of the traffic),
+                // relaxed to 64 for clouds too small to fill the device otherwise
+                long long cap = std::max(1, c.n / (total_blocks * 8 < capacity ? 64 : 512));
+                S = (int)std::max(1LL, std::min(want, cap));
+            }
+        }
+        S = std::max(1, std::min(S, c.n));
+        // slice length: a multiple of the sources one LDS tile gives all wave
+        // groups (each group takes its own kSubTile of every tile), so that
+        // only a cloud's last slice ends in a partial tile
+        int chunk = (c.n + S - 1) / S;
+        if (!exact) chunk = ((chunk + quantum - 1) / quantum) * quantum;
+        S = (c.n + chunk - 1) / chunk;
+        L.slices[ci] = S;
+        L.max_slices = std::max(L.max_slices, S);
+        for (int s = 0; s < S; ++s) {
+            const int j0 = c.off + s * chunk, j1 = c.off + std::min(c.n, (s + 1) * chunk);
+            for (int b = 0; b < ib; ++b) {
+                CoulombItem w{};
+                w.cloud = (int)ci;
+                w.i0 = c.off + b * blk;
+                w.i1 = c.off + std::min(c.n, (b + 1) * blk);
+                w.j0 = j0;
+                w.j1 = j1;
+                w.slice = s;
+                L.items.push_back(w);
+            }
+        }
+    }
+    // Work-group b runs on XCD b % 8 and takes the b / 8-th item of that
+    // XCD's contiguous run (xcd_item in the kernels); an XCD starts its
+    // work-groups in order.  Each run is therefore ordered costliest first
+    // (an item costs its source count: every lane loops over all of them),
+    // so that short items fill the last, partly empty round of the XCD; the
+    // runs themselves keep their clouds, which keeps each XCD's sources in
+    // its L2.
+    const int nb = (int)L.items.size(), qx = nb / 8, rx = nb % 8;
+    for (int x = 0; x < 8; ++x) {
+        const int b0 = x * qx + std::min(x, rx), cnt = qx + (x < rx ? 1 : 0);
+        std::stable_sort(L.items.begin() + b0, L.items.begin() + b0 + cnt,
+                         [](const CoulombItem &u, const CoulombItem &v) {
+                             return u.j1 - u.j0 > v.j1 - v.j0;
+                         });
+    }
+    return L;
+}
+
+std::string check_plan(const std::vector<PlanCloud> &clouds, const Plan &p)
+{
+    char buf[256];
+    if (p.slices.size() != clouds.size()) return "one slice count per cloud is needed";
+    for (size_t ci = 0; ci < clouds.size(); ++ci)
+        if (p.slices[ci] < 1 || p.slices[ci] > p.max_slices) {
+            std::snprintf(buf, sizeof(buf), "cloud %zu has %d slices, max_slices %d", ci,
+                          p.slices[ci], p.max_slices);
+            return buf;
+        }
+    // per cloud: pairs covered, summed over items, must be n^2 with no overlap
+    std::vector<long long> covered(clouds.size(), 0);
+    for (size_t k = 0; k < p.items.size(); ++k) {
+        const CoulombItem &w = p.items[k];
+        if (w.cloud < 0 || w.cloud >= (int)clouds.size()) return "item names no cloud";
+        const PlanCloud &c = clouds[w.cloud];
+        const int lo = c.off, hi = c.off + c.n;
+        if (!c.on || w.i0 < lo || w.i1 > hi || w.i0 >= w.i1 || w.j0 < lo || w.j1 > hi ||
+            w.j0 >= w.j1) {
+            std::snprintf(buf, sizeof(buf), "item %zu [%d,%d)x[%d,%d) leaves cloud %d [%d,%d)", k,
+                          w.i0, w.i1, w.j0, w.j1, w.cloud, lo, hi);
+            return buf;
+        }
+        if (w.slice < 0 || w.slice >= p.slices[w.cloud]) {
+            std::snprintf(buf, sizeof(buf), "item %zu writes slice %d of cloud %d with %d slices",
+                          k, w.slice, w.cloud, p.slices[w.cloud]);
+            return buf;
+        }
+        covered[w.cloud] += (long long)(w.i1 - w.i0) * (w.j1 - w.j0);
+    }
+    // overlap: items of one slice partition the targets, and the slices of
+    // one target block partition the sources
+    for (size_t ci = 0; ci < clouds.size(); ++ci) {
+        const PlanCloud &c = clouds[ci];
+        const long long want = (c.on && c.n > 0) ? (long long)c.n * c.n : 0;
+        if (covered[ci] != want) {
+            std::snprintf(buf, sizeof(buf), "cloud %zu: items cover %lld pairs of %lld", ci,
+                          covered[ci], want);
+            return buf;
+        }
+    }
+    std::vector<std::vector<int>> tgt(clouds.size()), src(clouds.size());
+    for (size_t ci = 0; ci < clouds.size(); ++ci) {
+        tgt[ci].assign((size_t)std::max(0, clouds[ci].n), 0);
+        src[ci].assign((size_t)std::max(0, clouds[ci].n), 0);
+    }
+    for (const CoulombItem &w : p.items) {
+        const int off = clouds[w.cloud].off;
+        if (w.slice == 0)
+            for (int i = w.i0; i < w.i1; ++i) tgt[w.cloud][i - off]++;
+        if (w.i0 == off)
+            for (int j = w.j0; j < w.j1; ++j) src[w.cloud][j - off]++;
+    }
+    for (size_t ci = 0; ci < clouds.size(); ++ci) {
+        if (!clouds[ci].on) continue;
+        for (int i = 0; i < clouds[ci].n; ++i)
+            if (tgt[ci][i] != 1 || src[ci][i] != 1) {
+                std::snprintf(buf, sizeof(buf), "cloud %zu ion %d: %d target blocks, %d slices", ci,
+                              i, tgt[ci][i], src[ci][i]);
+                return buf;
+            }
+    }
+    return std::string();
+}
+
+}  // namespace sbk

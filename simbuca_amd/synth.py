@@ -19,7 +19,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import AMU, EL_CHARGE, KB, Fields, default_fields
+from . import AMU, EL_CHARGE, KB, KE, Fields, default_fields
 
 
 def mass_kg(mass_amu):
@@ -148,3 +148,25 @@ def workload(index: int, n_override=None) -> Workload:
         return Workload("262,144 ions 133/85/39u, 10K, Kahan, 5us", [c], 5e-6, 1e-9, 1e-9, 1e-9,
                         "mixed_kahan", TIME_STAGE, {"seed": 5})
     raise ValueError(index)
+
+
+def cloud_invariants(c: Cloud, pos, vel):
+    """Total energy (kinetic + ideal Penning quadrupole + Coulomb) and
+    canonical angular momentum L_z = sum m (x v_y - y v_x) + q e B (x^2+y^2)/2
+    of cloud c at state (pos, vel), with the cloud's B and V0/d0.  Both are
+    conserved for an axisymmetric trap without drive or friction."""
+    B, V0, d0 = c.fields.Bz, c.fields.V0, c.fields.d0
+    cpen = V0 / (2 * d0 * d0) if V0 else 0.0
+    q = c.charge.astype(np.float64) * EL_CHARGE
+    kin = 0.5 * (c.mass[:, None] * vel * vel).sum()
+    trap = (q * cpen * (pos[:, 2] ** 2 - 0.5 * (pos[:, 0] ** 2 + pos[:, 1] ** 2))).sum()
+    coul = 0.0
+    for i in range(0, c.n, 512):
+        d = pos[i:i + 512, None, :] - pos[None, :, :]
+        r = np.sqrt((d * d).sum(-1))
+        idx = np.arange(i, min(i + 512, c.n))
+        r[idx - i, idx] = np.inf
+        coul += 0.5 * KE * (q[i:i + 512, None] * q[None, :] / r).sum()
+    lz = (c.mass * (pos[:, 0] * vel[:, 1] - pos[:, 1] * vel[:, 0])).sum() + \
+        (q * B * (pos[:, 0] ** 2 + pos[:, 1] ** 2) / 2).sum()
+    return kin + trap + coul, lz

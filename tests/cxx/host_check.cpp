@@ -18,11 +18,13 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <random>
 #include <string>
 #include <vector>
 
 #include "simbuca_hip.h"
 #include "../../oracle/simbuca_oracle.h"
+#include "../../simbuca_amd/csrc/sbk_plan.h"
 
 // ---- stubs of the engine (sbk_engine.cpp needs the device) ----------------
 namespace sbk {
@@ -275,6 +277,59 @@ static void check_oracle()
     }
 }
 
+
+// The Coulomb work list (sbk_plan.cpp) over cloud mixes and launch shapes:
+// check_plan must hold for every list (each item's slice is below its
+// cloud's slice count, which is at most max_slices, the number of partial
+// planes the engine allocates; every pair covered once), and must reject a
+// list with an item writing one plane past max_slices or a pair missing.
+static void check_planner()
+{
+    std::mt19937 mt(7u);
+    std::vector<std::vector<int>> mixes = {{1}, {0}, {500}, {10000}, {65536}, {1, 0, 2, 63, 64, 65},
+                                           {4096, 64, 4095, 257}};
+    for (int k = 0; k < 6; ++k) {   // ensembles with log-uniform sizes in [64, 4096]
+        std::uniform_real_distribution<double> u(std::log(64.0), std::log(4096.0));
+        std::vector<int> v(k == 0 ? 256 : 1 + (int)(mt() % 40));
+        for (int &n : v) n = (int)std::lround(std::exp(u(mt)));
+        mixes.push_back(v);
+    }
+    long long lists = 0;
+    for (const auto &sizes : mixes) {
+        std::vector<sbk::PlanCloud> pc(sizes.size());
+        int off = 0;
+        for (size_t i = 0; i < sizes.size(); ++i) {
+            pc[i].n = sizes[i];
+            pc[i].off = off;
+            pc[i].on = (i % 5) != 3;   // some clouds without Coulomb
+            off += sizes[i];
+        }
+        for (int q : {1, 4})
+            for (int tpt : {1, 2, 4, 8})
+                for (int sreq : {0, 1, 3, 7, 64, 100000})
+                    for (long long cap : {256LL, 1024LL, 4096LL})
+                        for (int exact : {0, 1}) {
+                            const int blk = sbk::kBlock / q * tpt;
+                            sbk::Plan p = sbk::plan_coulomb(pc, exact ? sbk::kBlock : blk,
+                                                            sbk::kSubTile * q, sreq, cap, exact != 0);
+                            const std::string bad = sbk::check_plan(pc, p);
+                            if (!bad.empty()) std::fprintf(stderr, "plan: %s\n", bad.c_str());
+                            CHECK(bad.empty());
+                            if (exact) CHECK(p.max_slices == 1);
+                            lists++;
+                            if (!p.items.empty() && lists % 7 == 0) {
+                                sbk::Plan q1 = p;
+                                q1.items[q1.items.size() / 2].slice = q1.max_slices;
+                                CHECK(!sbk::check_plan(pc, q1).empty());
+                                sbk::Plan q2 = p;
+                                q2.items.pop_back();
+                                CHECK(!sbk::check_plan(pc, q2).empty());
+                            }
+                        }
+    }
+    CHECK(lists > 1000);
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) {
@@ -287,6 +342,7 @@ int main(int argc, char **argv)
     check_ini(dir, argc, argv);
     check_writer(dir);
     check_oracle();
+    check_planner();
     if (failures) {
         std::fprintf(stderr, "host_check: %d failure(s)\n", failures);
         return 1;

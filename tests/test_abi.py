@@ -4,6 +4,7 @@ oracle.  No GPU work is launched here."""
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -31,10 +32,35 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
-def test_struct_sizes_match_header():
-    # sbk_config: 5 ints, padding to 8, 3 doubles (x86-64 ABI)
-    assert ctypes.sizeof(S.Config) == 5 * 4 + 4 + 3 * 8 + 2 * 4
-    assert ctypes.sizeof(S.Stats) == 7 * 8
+def _c_layout(tmp_path):
+    """sizeof and offsetof of the header's structs, as gcc lays them out."""
+    src = tmp_path / "layout.c"
+    exe = tmp_path / "layout"
+    fields = {"sbk_config": [n for n, _ in S.Config._fields_],
+              "sbk_cloud_status": [n for n, _ in S.CloudStatus._fields_],
+              "sbk_stats": [n for n, _ in S.Stats._fields_],
+              "sbk_fields": [n for n, _ in S.Fields._fields_]}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "simbuca_hip.h"', 'int main(void) {']
+    for t, names in fields.items():
+        lines.append(f'printf("{t} size %zu\\n", sizeof({t}));')
+        for n in names:
+            lines.append(f'printf("{t} {n} %zu\\n", offsetof({t}, {n}));')
+    lines.append("return 0; }")
+    src.write_text("\n".join(lines))
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    return {tuple(ln.split()[:2]): int(ln.split()[2]) for ln in out.splitlines()}
+
+
+def test_struct_layouts_match_header(tmp_path):
+    # every ctypes mirror has the C struct's size and field offsets
+    lay = _c_layout(tmp_path)
+    for t, cls in (("sbk_config", S.Config), ("sbk_cloud_status", S.CloudStatus),
+                   ("sbk_stats", S.Stats), ("sbk_fields", S.Fields)):
+        assert ctypes.sizeof(cls) == lay[(t, "size")], t
+        for n, _ in cls._fields_:
+            assert getattr(cls, n).offset == lay[(t, n)], (t, n)
     assert ctypes.sizeof(S.Fields) == ctypes.sizeof(O.Fields)
     for (a, ta), (b, tb) in zip(S.Fields._fields_, O.Fields._fields_):
         assert a == b and ta == tb

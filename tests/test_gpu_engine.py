@@ -281,7 +281,7 @@ def test_local_coords_keep_the_accuracy(exact_65536):
 
 @pytest.mark.parametrize("precision", ["mixed", "mixed_kahan"])
 @pytest.mark.parametrize("unit", [True, False])
-@pytest.mark.parametrize("tpt,groups", [(1, 4), (2, 4), (4, 4), (2, 1)])
+@pytest.mark.parametrize("tpt,groups", [(1, 4), (2, 4), (4, 4), (2, 1), (8, 4), (8, 1)])
 def test_local_coords_close_pairs_take_the_fp64_separation(rng, precision, unit, tpt, groups):
     # 16 pairs of ions 20 nm apart in a 5,003-ion cloud: each partner's pull
     # outweighs the rest of the cloud about 1e5 times, so the per-ion error of
@@ -308,7 +308,7 @@ def test_local_coords_close_pairs_take_the_fp64_separation(rng, precision, unit,
             assert per_ion.max() < 1e-6, (local, per_ion.max())
 
 
-@pytest.mark.parametrize("tpt", [1, 2, 4])
+@pytest.mark.parametrize("tpt", [1, 2, 4, 8])
 @pytest.mark.parametrize("groups", [1, 4])
 @pytest.mark.parametrize("slices", [0, 1, 3, 7])
 @pytest.mark.parametrize("local", [0, 1])
@@ -461,6 +461,40 @@ def test_adaptive_steps_match_reference_golden(use_graph):
     assert st["lifetime"] == pytest.approx(g["scalars"][0], rel=1e-9)
     assert np.allclose(pos, g["pos_out"], rtol=1e-8, atol=1e-8 * np.abs(g["pos_out"]).max())
     assert np.allclose(vel, g["vel_out"], rtol=1e-8, atol=1e-8 * np.abs(g["vel_out"]).max())
+
+
+@pytest.mark.parametrize("beta", [0.04, 0.08])
+def test_pi_controller_matches_oracle(beta):
+    """PI control (sbk_config.beta, the reference's documented switch,
+    bin/ode.cpp:427-432) on the golden 64-ion adaptive cloud: 200 steps with
+    the same accept/reject sequence and h history as the oracle, whose
+    controller takes the same beta."""
+    g = load_golden("steps_adaptive_n64.npz")
+    sf = golden_fields(g["fields"], S.default_fields)
+    of = golden_fields(g["fields"], O.fields)
+    m = O.mass_kg(g["mass_amu"])
+    kw = dict(rtol=float(g["rtol"]), atol=float(g["atol"]), h0=float(g["h0"]))
+    c = O.Cloud(g["pos"], g["vel"], m, g["charge"], of, lifetime=float(g["lifetime"]),
+                adaptive=True, beta=beta, **kw)
+    with S.Engine(precision="fp64_exact", adaptive=True, time_mode=S.TIME_FROZEN, beta=beta,
+                  **kw) as e:
+        e.add_cloud(g["pos"], g["vel"], m, g["charge"], sf, lifetime=float(g["lifetime"]))
+        gh, oh = [], []
+        for _ in range(200):
+            gh.append(e.status(0)["h"])
+            oh.append(c.state()[2]["h"])
+            e.step()
+            c.step()
+        pos, vel = e.state(0)
+        st = e.status(0)
+    op, ov, sc = c.state()
+    assert st["attempts"] == sc["attempts"] and st["steps"] == sc["steps"] == 200
+    assert np.allclose(gh, oh, rtol=1e-9, atol=0)
+    assert st["lifetime"] == pytest.approx(sc["lifetime"], rel=1e-9)
+    assert np.allclose(pos, op, rtol=1e-8, atol=1e-8 * np.abs(op).max())
+    assert np.allclose(vel, ov, rtol=1e-8, atol=1e-8 * np.abs(ov).max())
+    # and it is not the beta = 0 run: the h history differs
+    assert not np.allclose(gh, g["hist"][:200, 0], rtol=1e-6, atol=0)
 
 
 def test_single_attempt_stage_combination_bit_identical(rng):
@@ -739,6 +773,25 @@ def test_cunbody_abi_exact(rng, n):
     ai = S.cunbody1_force(x, mj, x.copy(), eps2, precision="fp64_exact")
     ref = O.cunbody_force(x, mj, x, eps2)
     assert np.array_equal(ai, ref)
+
+
+@pytest.mark.parametrize("n", [2, 257, 3000])
+def test_cunbody_abi_eps2_zero_masks_the_self_pair(rng, n):
+    """eps2 = 0 with the reference's call (xj = pos2, xi = a copy of it,
+    bin/cunbody.cpp:65-66): each ion meets itself at r = 0.  Every precision
+    must stay finite and equal the oracle, which skips that pair."""
+    x = rng.normal(0, 1e-3, (n, 3))
+    mj = np.ones(n)
+    ref = O.cunbody_force(x, mj, x.copy(), 0.0)
+    assert np.isfinite(ref).all()
+    for p, tol in (("fp64_exact", 0.0), ("fp64", 1e-13), ("mixed", 1e-6), ("mixed_kahan", 1e-6)):
+        print(f"cunbody eps2=0 n={n} {p}", flush=True)
+        ai = S.cunbody1_force(x, mj, x.copy(), 0.0, precision=p)
+        assert np.isfinite(ai).all(), p
+        if tol == 0.0:
+            assert np.array_equal(ai, ref), p
+        else:
+            assert _rows_error(ai, ref) < tol, p
 
 
 def test_cunbody_abi_fp64_and_mixed(rng):

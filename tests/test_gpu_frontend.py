@@ -342,3 +342,25 @@ def test_crossings_of_two_clouds_and_a_short_buffer(rng):
         assert list(sel["step"]) == [x[0] for x in r] and list(sel["ion"]) == [x[1] for x in r]
         assert np.array_equal(sel["t"], [x[2] for x in r])
         assert np.array_equal(sel["pos"], np.array([x[3] for x in r]).reshape(-1, 3))
+
+
+def test_empty_cloud_file_ends_and_writes_its_files(tmp_path, monkeypatch):
+    """NParticles = 0: the move loop is skipped, as move_particles leaves at
+    once for an empty cloud (bin/ion_fly.cpp:87-88); every operation, the
+    prints after each and the closing block still run.  (The reference
+    program itself ends with a segmentation fault on such a file, so there is
+    no reference output to compare with.)"""
+    print("run_ini empty cloud", flush=True)
+    run = tmp_path / "run"
+    run.mkdir()
+    with open(os.path.join(INI, "paul_cycle.ini")) as fh:
+        text = fh.read().replace("NParticles = 12", "NParticles = 0")
+    assert "NParticles = 0" in text
+    (run / "sim.ini").write_text(text)
+    monkeypatch.chdir(run)
+    S.run_ini("sim.ini", precision=S.FP64_EXACT, time_mode=S.TIME_FROZEN)
+    out = run / "out"
+    names = sorted(os.listdir(out))
+    assert "paul_cycle_pAll.txt" in names and "paul_cycle_logfile.txt" in names, names
+    rows = [ln for ln in open(out / "paul_cycle_pAll.txt") if ln[:1].isdigit()]
+    assert rows == []

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from conftest import golden_fields, load_golden
+from conftest import TRAP_KEYS, golden_fields, load_golden
 
 E = 1.60217733e-19
 KE = 8.987551787e9
@@ -131,6 +131,44 @@ def test_cunbody_interface_semantics(rng):
         r2 = (d * d).sum(1) + eps2
         want[i] = (mj[:, None] * d / r2[:, None] ** 1.5).sum(0)
     assert np.allclose(ai, want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
+
+
+def test_cunbody_interface_reproduces_the_golden_force():
+    """The reference's CUNBODY call (bin/cunbody.cpp:65-66: xj = pos2, mj = 1,
+    xi = a copy of pos2) scaled as its caller does, derivs_j -=
+    coulomb_scale e^2 ke / m_j * ai_j (bin/force.cpp:87-93), must give the
+    reference's own CPU force() for unit charges without trap or TOF terms
+    (force_n257_coulomb_scaled.npz, coulomb_scale 0.5).  Same summation order
+    per target; only where the prefactor multiplies differs, so the two agree
+    to a few ulps.  A dropped mass, a sign or a swapped xi/xj fails this."""
+    g = load_golden("force_n257_coulomb_scaled.npz")
+    assert (g["charge"] == 1).all()
+    fields = dict(zip(TRAP_KEYS, g["fields"].tolist()))
+    assert fields["trap"] == 0 and fields["tof"] == 0 and fields["coulomb"] == 1
+    m = O.mass_kg(g["mass_amu"])
+    x = np.ascontiguousarray(g["pos"])
+    scale = fields["coulomb_scale"] * E * E * KE / m
+    for eps2 in (0.0, float(np.float32(1e-23))):
+        ai = O.cunbody_force(x, np.ones(len(x)), x.copy(), eps2)
+        assert np.isfinite(ai).all()
+        derivs = -scale[:, None] * ai
+        err = np.linalg.norm(derivs - g["derivs"]) / np.linalg.norm(g["derivs"])
+        assert err < (1e-14 if eps2 == 0.0 else 1e-10), (eps2, err)
+    # a wrong sign or a missing mass fails
+    assert np.linalg.norm(scale[:, None] * ai - g["derivs"]) > 0.5 * np.linalg.norm(g["derivs"])
+
+
+def test_cunbody_interface_skips_the_self_pair_only_for_aliased_sets(rng):
+    x = rng.normal(0, 1e-3, (17, 3))
+    a0 = O.cunbody_force(x, np.ones(17), x.copy(), 0.0)
+    assert np.isfinite(a0).all()
+    # a different target set holding one of the sources: that pair is 0/0
+    xi = x[:5].copy()
+    a1 = O.cunbody_force(x, np.ones(17), xi, 0.0)
+    assert np.isnan(a1).all(axis=1).all()
+    a2 = O.cunbody_force(x, np.ones(17), xi, 1e-20)
+    assert np.isfinite(a2).all()
+    assert np.allclose(a2, a0[:5], rtol=1e-12, atol=1e-12 * np.abs(a0).max())
 
 
 # ---------------------------------------------------------------------------
@@ -256,6 +294,54 @@ def test_controller_closed_forms(err, reject, want_accept, want_scale):
     else:
         assert h == pytest.approx(1e-8 * want_scale, rel=1e-14)
         assert rj is True
+
+
+@pytest.mark.parametrize("beta", [0.04, 0.05, 0.08])
+def test_pi_controller_exponents(beta):
+    """PI control (the reference's documented switch, bin/ode.cpp:427-432):
+    log(hnext/h) = log 0.9 - alpha log err + beta log errold with
+    alpha = 0.2 - 0.75 beta, inside the clamps.  Scaling errold by k moves
+    hnext by exactly k^beta, and scaling err by k moves it by k^-alpha; k is
+    chosen so that each factor is 2.  A sign slip, beta on the wrong error or
+    an alpha that ignores beta fails; so does a reject that uses beta."""
+    alpha = 0.2 - 0.75 * beta
+    err, errold = 0.5, 0.05
+    k_old = 2.0 ** (1.0 / beta)            # errold x k_old  -> hnext x 2
+    k_err = 2.0 ** (-1.0 / alpha)          # err x k_err     -> hnext x 2
+    _, _, h0, _, _ = O.controller(err, 1e-8, 1e-8, errold / k_old ** 0.5, False, beta=beta)
+    _, _, h1, _, _ = O.controller(err, 1e-8, 1e-8, errold * k_old ** 0.5, False, beta=beta)
+    assert h1 / h0 == pytest.approx(2.0, rel=1e-13)
+    _, _, h2, eo, _ = O.controller(err * k_err, 1e-8, 1e-8, errold, False, beta=beta)
+    _, _, h3, _, _ = O.controller(err, 1e-8, 1e-8, errold, False, beta=beta)
+    assert h2 / h3 == pytest.approx(2.0, rel=1e-13)
+    assert eo == err * k_err                 # errold bookkeeping is beta-free
+    assert 0.2 < h3 / 1e-8 < 10.0 and 0.2 < h2 / 1e-8 < 10.0   # inside the clamps
+    # beta = 0 is the reference's controller exactly
+    assert O.controller(err, 1e-8, 1e-8, errold, False) == O.controller(err, 1e-8, 1e-8, 0.37, False, beta=0.0)
+    # a reject does not look at errold
+    r1 = O.controller(4.0, 1e-8, 1e-8, 1e-4, False, beta=beta)
+    r2 = O.controller(4.0, 1e-8, 1e-8, 0.9, False, beta=beta)
+    assert r1[:3] == r2[:3] and r1[4] == r2[4] and r1[0] is False
+    assert (r1[3], r2[3]) == (1e-4, 0.9)     # a reject leaves errold alone
+    assert r1[1] / 1e-8 == pytest.approx(max(0.9 * 4.0 ** -alpha, 0.2), rel=1e-14)
+
+
+def test_pi_controller_in_a_run_changes_the_step_sequence():
+    # the same cloud with beta 0 and 0.08: both reach t_end, with different step counts
+    rng = np.random.Generator(np.random.PCG64(3))
+    n = 24
+    pos = rng.normal(0, 2e-4, (n, 3)); vel = rng.normal(0, 100.0, (n, 3))
+    m = O.mass_kg(np.full(n, 133.0)); ch = np.ones(n, np.int32)
+    f = _fo(trap=1, Vrf=0.0, kappa=1.0, Vdc=5.0, z0=11e-3, r0=1.0, coulomb=1)
+    runs = {}
+    for beta in (0.0, 0.08):
+        c = O.Cloud(pos, vel, m, ch, f, h0=1e-9, rtol=1e-9, atol=1e-9, beta=beta)
+        c.advance(2e-6)
+        p, v, sc = c.state()
+        runs[beta] = (p, sc)
+    # (positions are not compared: each run overshoots t_end by its own last step)
+    assert runs[0.0][1]["steps"] != runs[0.08][1]["steps"]
+    assert min(runs[0.0][1]["lifetime"], runs[0.08][1]["lifetime"]) >= 2e-6
 
 
 def test_stepsize_underflow_raises():
