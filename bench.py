@@ -455,6 +455,24 @@ def main():
     e64.close()
     log(f"fp64: {fp64}")
 
+    # the faster FP64 kernel (hardware |r|^-3 estimate with a second-order
+    # correction, a few ulp per pair) on the same window: the comparator the
+    # mixed engine's speed-up is also stated against
+    er, dtr, str_, _ = run_engine(S, w, "fp64_rsq", k64, args.warmup, timing=True, tune=tune64)
+    fp64_rsq = {
+        "pair_interactions_per_s": str_["pair_interactions"] / dtr,
+        "ion_steps_per_s": str_["ion_steps"] / dtr,
+        "ms_per_step": 1e3 * dtr / k64,
+        "coulomb_ms_per_eval": str_["coulomb_ms"] / max(1.0, str_["pair_interactions"] / (float(n) * (n - 1))),
+        "steps": k64,
+        "accepted_steps": str_["steps"],
+    }
+    er.close()
+    for d in (fp64, fp64_rsq):
+        d["kernel_frac_of_fp64_peak"] = (FLOP_PER_PAIR * n * (n - 1) / (d["coulomb_ms_per_eval"] * 1e-3)
+                                         / 1e12 / PEAK_FP64_TFLOPS)
+    log(f"fp64_rsq: {fp64_rsq}")
+
     cun = cunbody_abi(S, w)
     log(f"cunbody1_force ABI {cun}")
 
@@ -511,7 +529,10 @@ def main():
         "attempts": st["attempts"],
         "graph_replay": graph,
         "fp64": fp64,
+        "fp64_rsq": fp64_rsq,
         "mixed_vs_fp64_ion_steps": ion_steps / fp64["ion_steps_per_s"] if fp64["ion_steps_per_s"] else None,
+        "mixed_vs_fp64_rsq_ion_steps": (ion_steps / fp64_rsq["ion_steps_per_s"]
+                                        if fp64_rsq["ion_steps_per_s"] else None),
         "force_error": ferr,
         "cunbody_abi": cun,
         "trajectory_deviation": traj,

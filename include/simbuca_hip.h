@@ -65,7 +65,8 @@ void cunbody1_force_end(double xj[][3], double mj[], double xi[][3], double eps2
 
 /* Arithmetic used by cunbody1_force* (no counterpart in the reference,
  * whose backend precision is fixed inside a closed binary): SBK_FP64
- * (default) or SBK_MIXED.  Returns the previous setting. */
+ * (default), SBK_FP64_EXACT, SBK_FP64_RSQ, SBK_MIXED or SBK_MIXED_KAHAN.
+ * Returns the previous setting; other values leave it unchanged. */
 int sbk_cunbody_set_precision(int precision);
 
 /* ------------------------------------------------------------------------
@@ -115,7 +116,10 @@ enum {
     SBK_FP64 = 1,        /* FP64, tiled and split over sources (fast FP64 baseline) */
     SBK_MIXED = 2,       /* FP64 separations rounded to FP32, FP32 rsqrt and tile sums,
                             FP64 accumulation across tiles */
-    SBK_MIXED_KAHAN = 3  /* as SBK_MIXED with compensated FP32 accumulation across tiles */
+    SBK_MIXED_KAHAN = 3, /* as SBK_MIXED with compensated FP32 accumulation across tiles */
+    SBK_FP64_RSQ = 4     /* as SBK_FP64, with |r|^-3 from the hardware reciprocal-square-root
+                            estimate and a second-order correction instead of IEEE sqrt and
+                            division (a few ulp per pair; no counterpart in the reference) */
 };
 
 /* Time at which force() is evaluated inside a step. */
@@ -167,7 +171,7 @@ typedef struct sbk_fields {
 } sbk_fields;
 
 typedef struct sbk_config {
-    int    precision;   /* SBK_FP64_EXACT .. SBK_MIXED_KAHAN */
+    int    precision;   /* SBK_FP64_EXACT .. SBK_FP64_RSQ */
     int    time_mode;   /* SBK_TIME_FROZEN or SBK_TIME_STAGE */
     int    adaptive;    /* 1: error_succes controller; 0: fixed step (bin/ode.cpp:478-486) */
     int    use_graph;   /* 1: replay one captured hipGraph per attempt */
@@ -292,7 +296,7 @@ int  sbk_eval_coulomb(sbk_engine *e, int cloud, int precision, double *out);
  * 1 lets far sources form separations from FP32 offsets, 0 keeps
  * FP64-formed separations everywhere, -1 (automatic) chooses 1 when the
  * clouds' pair-weighted mean size is at least 16384 ions.  Applies to
- * SBK_FP64 and SBK_MIXED*; SBK_FP64_EXACT always uses one thread per target
+ * SBK_FP64, SBK_FP64_RSQ and SBK_MIXED*; SBK_FP64_EXACT always uses one thread per target
  * and one slice. */
 int  sbk_set_launch(sbk_engine *e, int targets_per_thread, int source_groups, int slices,
                     int local_coords);

@@ -22,8 +22,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsimbuca_hip.so")
 
-FP64_EXACT, FP64, MIXED, MIXED_KAHAN = 0, 1, 2, 3
-PRECISIONS = {"fp64_exact": FP64_EXACT, "fp64": FP64, "mixed": MIXED, "mixed_kahan": MIXED_KAHAN}
+FP64_EXACT, FP64, MIXED, MIXED_KAHAN, FP64_RSQ = 0, 1, 2, 3, 4
+PRECISIONS = {"fp64_exact": FP64_EXACT, "fp64": FP64, "mixed": MIXED, "mixed_kahan": MIXED_KAHAN,
+              "fp64_rsq": FP64_RSQ}
 TIME_FROZEN, TIME_STAGE = 0, 1
 
 SBK_EUNDERFLOW = -4

@@ -161,7 +161,7 @@ int sbk_cunbody_set_precision(int precision)
 {
     const int old = g.precision;
     if (precision == SBK_FP64_EXACT || precision == SBK_FP64 || precision == SBK_MIXED ||
-        precision == SBK_MIXED_KAHAN)
+        precision == SBK_MIXED_KAHAN || precision == SBK_FP64_RSQ)
         g.precision = precision;
     return old;
 }

@@ -708,7 +708,8 @@ int launch_coulomb_timed(sbk_engine *e, int precision, const sbk::CoulombArgs &a
         HIP_TRY(hipEventRecord(evp->first, e->stream));
     }
     sbk::CoulombArgs b = a;
-    if (precision == SBK_FP64) b.eps_d = 1e-40;   // the guard of bin/force.cpp:136
+    if (precision == SBK_FP64 || precision == SBK_FP64_RSQ)
+        b.eps_d = 1e-40;   // the guard of bin/force.cpp:136
     b.local_coords = local_coords(e, precision);
     if (sbk::launch_coulomb(kp(precision), targets_per_thread(e, precision), e->groups, b, e->stream))
         return fail(SBK_EHIP, "Coulomb kernel launch failed: %s",
@@ -934,7 +935,7 @@ int sbk_device_info(char *name, int name_len, int *cus, int *clock_mhz)
 int sbk_create(const sbk_config *cfg, sbk_engine **out)
 {
     if (!cfg || !out) return fail(SBK_EINVAL, "null argument");
-    if (cfg->precision < SBK_FP64_EXACT || cfg->precision > SBK_MIXED_KAHAN)
+    if (cfg->precision < SBK_FP64_EXACT || cfg->precision > SBK_FP64_RSQ)
         return fail(SBK_EINVAL, "unknown precision %d", cfg->precision);
     if (cfg->time_mode != SBK_TIME_FROZEN && cfg->time_mode != SBK_TIME_STAGE)
         return fail(SBK_EINVAL, "unknown time mode %d", cfg->time_mode);
@@ -1337,7 +1338,7 @@ static int eval_common(sbk_engine *e, int cloud, int precision, int with_fields,
     int rc = prepare(e);
     if (rc) return rc;
     if (!out || cloud < 0 || cloud >= (int)e->clouds.size()) return fail(SBK_EINVAL, "bad cloud id");
-    if (precision < SBK_FP64_EXACT || precision > SBK_MIXED_KAHAN)
+    if (precision < SBK_FP64_EXACT || precision > SBK_FP64_RSQ)
         return fail(SBK_EINVAL, "unknown precision %d", precision);
     const HostCloud &c = e->clouds[cloud];
     ItemList *L = nullptr;
@@ -1406,7 +1407,7 @@ static int time_coulomb_pass(sbk_engine *e, int precision, int reps, bool with_s
     if (rc) return rc;
     if ((rc = check_partials(e, L))) return rc;
     sbk::CoulombArgs a = coulomb_args(e, *L, e->dev.P0, false);
-    if (precision == SBK_FP64) a.eps_d = 1e-40;
+    if (precision == SBK_FP64 || precision == SBK_FP64_RSQ) a.eps_d = 1e-40;
     a.local_coords = local_coords(e, precision);
     sbk::EngineDev d = e->dev;
     d.slices = L->d_slices;
@@ -1435,7 +1436,7 @@ int sbk_time_coulomb(sbk_engine *e, int precision, int reps, double *ms_per_laun
     int rc = prepare(e);
     if (rc) return rc;
     if (reps <= 0 || !ms_per_launch) return fail(SBK_EINVAL, "reps must be > 0");
-    if (precision < SBK_FP64_EXACT || precision > SBK_MIXED_KAHAN)
+    if (precision < SBK_FP64_EXACT || precision > SBK_FP64_RSQ)
         return fail(SBK_EINVAL, "unknown precision %d", precision);
     return time_coulomb_pass(e, precision, reps, false, ms_per_launch);
 }

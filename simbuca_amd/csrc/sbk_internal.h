@@ -137,7 +137,9 @@ struct EngineDev {
 int record_error(int code, const char *msg);
 
 // host-side launchers (sbk_kernels.hip)
-enum KernelPrecision { KP_FP64_EXACT = 0, KP_FP64 = 1, KP_MIXED = 2, KP_MIXED_KAHAN = 3 };
+enum KernelPrecision {
+    KP_FP64_EXACT = 0, KP_FP64 = 1, KP_MIXED = 2, KP_MIXED_KAHAN = 3, KP_FP64_RSQ = 4
+};
 
 // q: wave groups of a work-group that split the sources of one target set (1 or 4)
 int launch_coulomb(int precision, int targets_per_thread, int q, const CoulombArgs &a,

@@ -97,6 +97,9 @@ __global__ __launch_bounds__(kBlock) void coulomb_fp64_exact(CoulombArgs a)
 // and the sum over kSubTile sources are FP32; the sub-tile sums are folded
 // into FP64 (KAHAN: compensated FP32) accumulators.
 // FP64: the same tiling in FP64 with 1/sqrt.
+// FP64_RSQ: as FP64, with |r|^-3 from the hardware reciprocal-square-root
+// estimate (v_rsq_f64) and a second-order correction (rsq3_f64) instead of
+// the IEEE square root and division.
 // LOC (mixed modes only): the work-group takes the centre o of its targets'
 // bounding box (half-diagonal h) as origin, and each source also enters LDS
 // as the FP32 offset x_s - o (one FP64 subtraction per source per
@@ -128,15 +131,33 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
+// x^(-3/2) for normal x > 0 to within a few ulp, from the hardware
+// reciprocal-square-root estimate y = v_rsq_f64(x) = x^(-1/2) (1 + d):
+// with r = 1 - x y^2 (one fma, exact up to the rounding of y^2),
+// x^(-3/2) = y^3 (1 - r)^(-3/2) = y^3 (1 + 3r/2 + 15r^2/8 + 35r^3/16 + ...).
+// Keeping the terms to r^2 leaves a relative error of 35|r|^3/16 with
+// |r| ~ 2|d|: below 2^-60 for an estimate good to 2^-22, so the result is
+// set by the roundings of y^2, y^3 and the last fma.
+__device__ __forceinline__ double rsq3_f64(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    const double y2 = y * y;
+    const double w0 = y2 * y;
+    const double r = fma(-x, y2, 1.0);
+    const double c = r * fma(r, 1.875, 1.5);
+    return fma(w0, c, w0);
+}
+
 template <int R, int MODE, bool UNIT, int Q, bool LOC>
 __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
 {
-    static_assert(!LOC || MODE != KP_FP64, "local coordinates are a mixed-precision variant");
+    constexpr bool F64 = MODE == KP_FP64 || MODE == KP_FP64_RSQ;
+    static_assert(!LOC || !F64, "local coordinates are a mixed-precision variant");
     constexpr int L = kBlock / Q;          // lanes that share one target set
     constexpr int SPAN = kBlock / Q;       // sources of a tile handled by one wave group
     __shared__ double lx[kBlock], ly[kBlock], lz[kBlock];
     __shared__ float lqf[(LOC || UNIT) ? 1 : kBlock];
-    __shared__ double lqd[(MODE == KP_FP64) ? kBlock : 1];
+    __shared__ double lqd[F64 ? kBlock : 1];
     __shared__ float4 lf[LOC ? kBlock : 1];                 // LOC: (x_s - o, weight)
     __shared__ int lfar[LOC ? kBlock / kSubTile : 1];       // LOC: far sources per sub-tile
 
@@ -247,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
             lx[tid] = a.sx[t0 + tid];
             ly[tid] = a.sy[t0 + tid];
             lz[tid] = a.sz[t0 + tid];
-            if (MODE == KP_FP64)
+            if (F64)
                 lqd[tid] = a.sqd[t0 + tid];
             else if (!UNIT)
                 lqf[tid] = a.sqf[t0 + tid];
@@ -256,15 +277,20 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
         // this wave group's share of the tile
         const int b0 = grp * SPAN, b1 = min(cnt, b0 + SPAN);
 
-        if (MODE == KP_FP64) {
+        if (F64) {
             for (int j = b0; j < b1; ++j) {
                 const double sx = lx[j], sy = ly[j], sz = lz[j], q = lqd[j];
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const double dx = sx - px[r], dy = sy - py[r], dz = sz - pz[r];
                     const double d2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, epsd)));
-                    const double inv = 1.0 / sqrt(d2);
-                    const double w = q * (inv * inv) * inv;
+                    double w;
+                    if (MODE == KP_FP64_RSQ) {
+                        w = q * rsq3_f64(d2);
+                    } else {
+                        const double inv = 1.0 / sqrt(d2);
+                        w = q * (inv * inv) * inv;
+                    }
                     Ax[r] = fma(dx, w, Ax[r]);
                     Ay[r] = fma(dy, w, Ay[r]);
                     Az[r] = fma(dz, w, Az[r]);
@@ -811,6 +837,8 @@ static const void *tiled_fn_q(int precision, int loc)
     switch (precision) {
     case KP_FP64:
         return loc ? nullptr : reinterpret_cast<const void *>(&coulomb_tiled<R, KP_FP64, false, Q, false>);
+    case KP_FP64_RSQ:
+        return loc ? nullptr : reinterpret_cast<const void *>(&coulomb_tiled<R, KP_FP64_RSQ, false, Q, false>);
     case KP_MIXED:
         return loc ? reinterpret_cast<const void *>(&coulomb_tiled<R, KP_MIXED, UNIT, Q, true>)
                    : reinterpret_cast<const void *>(&coulomb_tiled<R, KP_MIXED, UNIT, Q, false>);

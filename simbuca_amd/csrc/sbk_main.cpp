@@ -2,7 +2,7 @@
 // simulation file on the GPU, as the reference's `simbuca <file.ini>` runs it
 // on the CPU (bin/main.cpp:13-31).
 //
-//   simbuca_hip [--precision fp64_exact|fp64|mixed|mixed_kahan]
+//   simbuca_hip [--precision fp64_exact|fp64|fp64_rsq|mixed|mixed_kahan]
 //               [--time frozen|stage] [--ame FILE] [--pdg FILE] [--quiet]
 //               [--describe] FILE.ini
 //
@@ -31,7 +31,7 @@ bool exists(const std::string &p)
 int usage()
 {
     std::fprintf(stderr,
-                 "usage: simbuca_hip [--precision fp64_exact|fp64|mixed|mixed_kahan] [--time frozen|stage]\n"
+                 "usage: simbuca_hip [--precision fp64_exact|fp64|fp64_rsq|mixed|mixed_kahan] [--time frozen|stage]\n"
                  "                   [--ame FILE] [--pdg FILE] [--quiet] [--describe] FILE.ini\n");
     return 2;
 }
@@ -57,6 +57,7 @@ int main(int argc, char **argv)
             else if (v == "fp64") precision = SBK_FP64;
             else if (v == "mixed") precision = SBK_MIXED;
             else if (v == "mixed_kahan") precision = SBK_MIXED_KAHAN;
+            else if (v == "fp64_rsq") precision = SBK_FP64_RSQ;
             else return usage();
         } else if (a == "--time") {
             if (!value(v)) return usage();

@@ -221,7 +221,8 @@ def exact_65536():
     return exact
 
 
-@pytest.mark.parametrize("precision,local,tol", [("fp64", 0, 1e-13), ("mixed", 0, 1e-6),
+@pytest.mark.parametrize("precision,local,tol", [("fp64", 0, 1e-13), ("fp64_rsq", 0, 1e-14),
+                                                 ("mixed", 0, 1e-6),
                                                  ("mixed", 1, 1e-6), ("mixed_kahan", 0, 1e-6),
                                                  ("mixed_kahan", 1, 1e-6)])
 def test_coulomb_precisions_at_65536(exact_65536, precision, local, tol):
@@ -323,6 +324,31 @@ def test_launch_shapes_agree(rng, tpt, groups, slices, local):
         exact = e.eval_coulomb(0, "fp64_exact")
     assert _rows_error(fp64, exact) < 1e-13
     assert _rows_error(mixed, exact) < 1e-6
+    with S.Engine(precision="fp64_rsq") as e:
+        e.add_cloud(pos, vel, m, ch, sf)
+        e.set_launch(tpt, groups, slices, 0)
+        rsq = e.eval_coulomb(0, "fp64_rsq")
+    assert _rows_error(rsq, exact) < 1e-14
+
+
+@pytest.mark.parametrize("n", [1, 2, 257, 4099])
+def test_fp64_rsq_against_the_oracle(rng, n):
+    """The FP64 kernel with the hardware |r|^-3 estimate and its second-order
+    correction: within 1e-14 normwise of the oracle's FP64 sum
+    (bin/force.cpp:147-175), charges and masses mixed; the self pair adds 0."""
+    amu, m, ch, pos, vel = _rand_cloud(rng, n, sigma=1e-3)
+    sf, of = _both_fields(trap=0, coulomb=1)
+    ref = O.force(pos, vel, m, ch, of, 0.0)
+    with S.Engine(precision="fp64_rsq") as e:
+        e.add_cloud(pos, vel, m, ch, sf)
+        f = e.eval_force(0)
+        st = e.status(0)
+    assert np.isfinite(f).all()
+    if n == 1:
+        assert np.array_equal(f, ref)
+        return
+    assert _rows_error(f, ref) < 1e-14, _rows_error(f, ref)
+    assert st["error"] == 0
 
 
 def test_autotune_table_round_trip(rng, tmp_path):

@@ -61,7 +61,7 @@ def test_workload1_first_200_steps_match_the_oracle():
     assert np.allclose(gv, ov, rtol=1e-9, atol=1e-9 * np.abs(ov).max())
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "fp64_rsq", "mixed"])
 def test_workload1_full_100us_conserves_energy_and_lz(precision):
     w, c = _workload()
     print(f"workload 1: 100 us, {precision}", flush=True)
