@@ -69,6 +69,20 @@ void cunbody1_force_end(double xj[][3], double mj[], double xi[][3], double eps2
  * Returns the previous setting; other values leave it unchanged. */
 int sbk_cunbody_set_precision(int precision);
 
+/* cunbody1_force with no device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, result and precision setting as
+ * cunbody1_force, but every device buffer (staging, slice partials, work
+ * items) lives in one static device array of the library, about 17 MB per
+ * GPU, held until the process ends; ni and nj are at most 32768 and larger
+ * sets return SBK_EINVAL without touching ai.  Runs on the null stream and
+ * returns when ai is written; 0 or an SBK_E* code (sbk_last_error() gives
+ * the message).  All calls on one GPU share its array, so calls that overlap
+ * on one GPU (from several host threads) race. */
+int sbk_cunbody_force_static(double xj[][3], double mj[], double xi[][3], double eps2,
+                             double ai[][3], int ni, int nj);
+
 /* ------------------------------------------------------------------------
  * Export set 1b -- reference NBODY interface, as its caller declares it
  * (bin/nbody.cpp:12-18; the reference's own kernel file defines different,

@@ -120,6 +120,7 @@ _lib = None
 # every symbol include/simbuca_hip.h declares
 EXPORTS = [
     "cunbody1_force", "cunbody1_force_begin", "cunbody1_force_end", "sbk_cunbody_set_precision",
+    "sbk_cunbody_force_static",
     "initialize", "finalize", "simulation_Async", "simulation_Async_begin", "simulation_Async_end",
     "sbk_default_config", "sbk_default_fields", "sbk_create", "sbk_destroy", "sbk_add_cloud",
     "sbk_set_fields", "sbk_set_step", "sbk_advance", "sbk_advance_until", "sbk_step", "sbk_attempts",
@@ -151,6 +152,7 @@ def lib():
         "cunbody1_force_begin": (None, [dp, dp, dp, c_dbl, dp, c_int, c_int]),
         "cunbody1_force_end": (None, [dp, dp, dp, c_dbl, dp, c_int, c_int]),
         "sbk_cunbody_set_precision": (c_int, [c_int]),
+        "sbk_cunbody_force_static": (c_int, [dp, dp, dp, c_dbl, dp, c_int, c_int]),
         "initialize": (None, [c_int, ctypes.c_bool]),
         "finalize": (ctypes.c_float, []),
         "simulation_Async": (None, [vp, vp, c_int, c_int, c_int, c_int]),
@@ -615,6 +617,27 @@ def cunbody1_force(xj, mj, xi, eps2, precision=None):
         old = L.sbk_cunbody_set_precision(p)
     try:
         L.cunbody1_force(_dp(xj), _dp(mj), _dp(xi), float(eps2), _dp(ai), xi.shape[0], xj.shape[0])
+    finally:
+        if old is not None:
+            L.sbk_cunbody_set_precision(old)
+    return ai
+
+
+def cunbody_force_static(xj, mj, xi, eps2, precision=None):
+    """cunbody1_force with no device allocation (sbk_cunbody_force_static):
+    at most 32768 targets and sources.  Raises on an error."""
+    L = lib()
+    xj = np.ascontiguousarray(xj, dtype=np.float64).reshape(-1, 3)
+    xi = np.ascontiguousarray(xi, dtype=np.float64).reshape(-1, 3)
+    mj = np.ascontiguousarray(mj, dtype=np.float64).ravel()
+    ai = np.zeros_like(xi)
+    old = None
+    if precision is not None:
+        p = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        old = L.sbk_cunbody_set_precision(p)
+    try:
+        _check(L.sbk_cunbody_force_static(_dp(xj), _dp(mj), _dp(xi), float(eps2), _dp(ai),
+                                          xi.shape[0], xj.shape[0]))
     finally:
         if old is not None:
             L.sbk_cunbody_set_precision(old)
