@@ -36,14 +36,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOP_PER_PAIR = 20.0          # GPU Gems 3 n-body convention
-# HBM bytes per launch of the mixed Coulomb kernel, from rocprofv3 FETCH_SIZE (x2.0) and
-# WRITE_SIZE (x1.0) counter runs (profiles/coulomb_mixed_65536_counters.md), keyed by
-# (architecture, ions, launch shape)
-# HBM bytes per launch of the mixed Coulomb kernel at 65,536 ions, from rocprofv3
-# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB device copy); the
-# automatic split of (4, 4, 0, 1) gives the same 32 slices as (4, 4, 32, 1)
-MEASURED_TRAFFIC = {("gfx950", 65536, (4, 4, 0, 1)): 17.65e6 + 50.33e6,
-                    ("gfx950", 65536, (4, 4, 32, 1)): 17.65e6 + 50.33e6}
+# HBM bytes per launch of the mixed Coulomb kernel at 65,536 ions with the table's shape
+# (4 targets/thread, 4 streams, 32 slices, local coordinates, streamed kernel), from
+# rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 in KiB per dispatch (the factors of a
+# 1 GiB device copy); the writes are the 32 slice partials, 32 x 3 x 8 B x 65,536
+MEASURED_TRAFFIC = {("gfx950", 65536, (4, 4, 32, 1, 1)): 2.0 * 8610.1 * 1024 + 1.0 * 49152.0 * 1024}
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector peak
 PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector peak
 HBM_PEAK_GBS = 8000.0
@@ -67,7 +64,8 @@ def tuned_shape(S, w, precision):
         shape = e.autotune(precision)
     finally:
         e.close()
-    log(f"autotuned launch shape (targets/thread, source groups, slices, local coords) = {shape}")
+    log(f"autotuned launch shape (targets/thread, source groups, slices, local coords, kernel) = "
+        f"{shape}")
     return shape
 
 
@@ -78,14 +76,15 @@ def best_table_shape(table, device, w, precision):
     with open(table) as fh:
         for line in fh:
             f = line.split()
-            if len(f) not in (8, 9) or f[0] != device:
+            if len(f) not in (8, 9, 10) or f[0] != device:
                 continue
             if int(f[1]) != n or int(f[2]) != len(w.clouds) or int(f[3]) != PRECISIONS[precision]:
                 continue
             ms = float(f[-1])
-            local = int(f[7]) if len(f) == 9 else 0
+            local = int(f[7]) if len(f) >= 9 else 0
+            kernel = int(f[8]) if len(f) == 10 else 0
             if best is None or ms < best[0]:
-                best = (ms, (int(f[4]), int(f[5]), int(f[6]), local))
+                best = (ms, (int(f[4]), int(f[5]), int(f[6]), local, kernel))
     return best[1] if best else None
 
 
@@ -95,7 +94,7 @@ def make_engine(S, w, precision, timing, use_graph=True, tune=None):
     for c in w.clouds:
         e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
     if tune is not None:
-        e.set_launch(*tune)
+        e.set_shape(tune)
     return e
 
 
@@ -114,10 +113,21 @@ def device_sync():
         raise RuntimeError("hipDeviceSynchronize failed")
 
 
+def spin_up(e, precision, ms=100.0):
+    """About `ms` of the engine's own Coulomb launches, untimed: the GPU's
+    clocks fall while the host prepares a run and take some 40 launches of
+    the 65,536-ion kernel (about 50 ms) to come back up (kernel traces show
+    the first launches 20% slower), which a short warm-up does not cover."""
+    first = e.time_coulomb(precision, 1)
+    e.time_coulomb(precision, max(1, min(2000, int(ms / max(first, 1e-3)))))
+
+
 def run_engine(S, w, precision, steps, warmup, timing, use_graph=True, tune=None):
-    """`warmup` then `steps` timed attempts of every cloud (inputs resident)."""
+    """`warmup` then `steps` timed attempts of every cloud (inputs resident),
+    after the clocks are spun up."""
     e = make_engine(S, w, precision, timing, use_graph, tune)
-    e.attempts(warmup)                  # uploads, activates with no end time, warms up
+    spin_up(e, precision)
+    e.attempts(warmup)                  # activates with no end time, warms up
     e.synchronize()
     e.reset_stats()
     device_sync()
@@ -129,6 +139,39 @@ def run_engine(S, w, precision, steps, warmup, timing, use_graph=True, tune=None
     st = e.stats()
     status = e.status(0)
     return e, dt, st, status
+
+
+def pi_control(S, w, tune, warmup, attempts=200):
+    """The reference's PI switch of the step controller (bin/ode.cpp:427-432),
+    beta = 0 (the reference's value), 0.04 and 0.08, on the same window of
+    `attempts` attempts from the same state: accepted fraction and ion-steps/s."""
+    out = {"attempts": attempts}
+    n = sum(c.n for c in w.clouds)
+    for beta in (0.0, 0.04, 0.08):
+        e = S.Engine(precision="mixed", time_mode=w.time_mode, rtol=w.rtol, atol=w.atol, h0=w.h0,
+                     beta=beta)
+        for c in w.clouds:
+            e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
+        if tune is not None:
+            e.set_shape(tune)
+        spin_up(e, "mixed")
+        e.attempts(warmup)
+        e.synchronize()
+        e.reset_stats()
+        t_start = e.status(0)["lifetime"]
+        device_sync()
+        t0 = time.perf_counter()
+        e.attempts(attempts, open_ended=False)
+        e.synchronize()
+        device_sync()
+        dt = time.perf_counter() - t0
+        st = e.stats()
+        advanced = e.status(0)["lifetime"] - t_start
+        e.close()
+        out[f"beta_{beta:g}"] = {"accepted": st["steps"], "accept_fraction": st["steps"] / attempts,
+                                 "ion_steps_per_s": st["ion_steps"] / dt,
+                                 "simulated_s": advanced, "simulated_s_per_wall_s": advanced / dt}
+    return out
 
 
 def cunbody_abi(S, w, calls=3):
@@ -241,7 +284,7 @@ def trajectory_deviation(S, w, tune, tune64, steps=100):
         e = S.Engine(precision=p, time_mode=w.time_mode, adaptive=False, h0=1e-9)
         e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
         if sh is not None:
-            e.set_launch(*sh)
+            e.set_shape(sh)
         e.attempts(steps)
         e.synchronize()
         res[p] = e.state(0)[0]
@@ -296,8 +339,10 @@ def _cpu_model():
 
 
 def _device_copy(nbytes, reps):
-    """hipMemcpy device-to-device of `nbytes`, `reps` times (a blit kernel that
-    reads and writes exactly nbytes: the calibration for FETCH_SIZE/WRITE_SIZE)."""
+    """hipMemcpy device-to-device of `nbytes`, `reps` times: the calibration of
+    FETCH_SIZE/WRITE_SIZE.  Run with HSA_ENABLE_SDMA=0 in the environment so
+    that the runtime copies with its blit kernel, which reads and writes
+    exactly nbytes, instead of the DMA engine, which counters do not see."""
     import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
     src, dst = ctypes.c_void_p(), ctypes.c_void_p()
@@ -320,7 +365,7 @@ def kernels_only(S, w, shape=None):
     for c in w.clouds:
         e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
     for p, reps in ((mixed, 5), ("fp64", 2)):
-        e.set_launch(*(shape or tuned_shape(S, w, p) or (0, 4, 0)))
+        e.set_shape(shape or tuned_shape(S, w, p) or (0, 4, 0, -1))
         log(f"{p} coulomb {e.time_coulomb(p, reps):.4f} ms/launch")
     e.close()
     _device_copy(1 << 30, 3)
@@ -371,8 +416,9 @@ def main():
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="skip the autotuner and use the automatic launch shape")
     ap.add_argument("--shape", default="",
-                    help="fixed Coulomb launch shape R,Q,S[,L] (targets/thread, source groups, "
-                         "slices, local coordinates -1/0/1); skips the autotuner")
+                    help="fixed Coulomb launch shape R,Q,S[,L[,K]] (targets/thread, source groups, "
+                         "slices, local coordinates -1/0/1, kernel -1 auto/0 tiled/1 streamed); "
+                         "skips the autotuner")
     ap.add_argument("--n", type=int, default=0, help="override the ion count (testing)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the Coulomb kernels and a 1 GiB device copy (profiler runs)")
@@ -476,6 +522,11 @@ def main():
     cun = cunbody_abi(S, w)
     log(f"cunbody1_force ABI {cun}")
 
+    pi = None
+    if not args.quick:
+        pi = pi_control(S, w, tune, args.warmup)
+        log(f"PI control {pi}")
+
     traj = inv = None
     if not args.quick:
         traj = trajectory_deviation(S, w, tune, tune64)
@@ -535,6 +586,7 @@ def main():
                                         if fp64_rsq["ion_steps_per_s"] else None),
         "force_error": ferr,
         "cunbody_abi": cun,
+        "pi_control": pi,
         "trajectory_deviation": traj,
         "invariant_drift": inv,
         "roofline": roofline,

@@ -314,21 +314,31 @@ int  sbk_eval_coulomb(sbk_engine *e, int cloud, int precision, double *out);
  * and one slice. */
 int  sbk_set_launch(sbk_engine *e, int targets_per_thread, int source_groups, int slices,
                     int local_coords);
+/* Coulomb kernel of the mixed precisions with 4 source groups and 1, 2 or 4
+ * targets per thread: SBK_KERNEL_TILED (the work-group's waves share each
+ * LDS source tile, a barrier per tile), SBK_KERNEL_STREAM (each wave streams
+ * its own source sub-tiles and loads the next one ahead, with no barrier in
+ * the source loop), or SBK_KERNEL_AUTO (the default).  Results are the same
+ * to rounding; only the speed differs.  Other launches use the tiled kernel. */
+enum { SBK_KERNEL_AUTO = -1, SBK_KERNEL_TILED = 0, SBK_KERNEL_STREAM = 1 };
+int  sbk_set_coulomb_kernel(sbk_engine *e, int kernel);
 /* Times `reps` launches of the Coulomb kernel of the current state's
  * positions for every cloud; ms_per_launch receives the mean. */
 int  sbk_time_coulomb(sbk_engine *e, int precision, int reps, double *ms_per_launch);
 /* Tries the launch shapes for this engine's clouds and keeps the fastest,
  * timing each Coulomb launch together with one pass that reads its slice
- * partials back (as every stage does); returns the chosen shape.  One line
- * per shape tried ("device ions clouds precision targets_per_thread
- * source_groups slices local_coords ms") is appended to `table_path` if
- * non-null. */
+ * partials back (as every stage does); returns the chosen shape and Coulomb
+ * kernel (SBK_KERNEL_TILED or SBK_KERNEL_STREAM; the streamed kernel is
+ * tried for the mixed precisions).  One line per shape tried ("device ions
+ * clouds precision targets_per_thread source_groups slices local_coords
+ * kernel ms") is appended to `table_path` if non-null. */
 int  sbk_autotune(sbk_engine *e, int precision, int *targets_per_thread, int *source_groups,
-                  int *slices, int *local_coords, const char *table_path);
+                  int *slices, int *local_coords, int *kernel, const char *table_path);
 
 /* Applies the fastest shape recorded in an sbk_autotune table for this
  * device, ion count, cloud count and precision, without timing anything
- * (lines without the local_coords column are read as local_coords 0):
+ * (lines without the kernel column are read as the tiled kernel, lines
+ * without the local_coords column as local_coords 0):
  * the counterpart of the reference's per-GPU launch table read at start-up
  * (bin/nbody.cpp:194-203,275-321).  Returns SBK_ESTATE when the table has
  * no matching line (the automatic shape stays in use). */

@@ -125,7 +125,8 @@ EXPORTS = [
     "sbk_default_config", "sbk_default_fields", "sbk_create", "sbk_destroy", "sbk_add_cloud",
     "sbk_set_fields", "sbk_set_step", "sbk_advance", "sbk_advance_until", "sbk_step", "sbk_attempts",
     "sbk_synchronize", "sbk_reorder", "sbk_set_crossing_plane", "sbk_take_crossings", "sbk_get_state", "sbk_get_status", "sbk_get_stats", "sbk_reset_stats",
-    "sbk_eval_force", "sbk_eval_coulomb", "sbk_set_launch", "sbk_time_coulomb", "sbk_autotune",
+    "sbk_eval_force", "sbk_eval_coulomb", "sbk_set_launch", "sbk_set_coulomb_kernel",
+    "sbk_time_coulomb", "sbk_autotune",
     "sbk_load_tuning", "sbk_spatial_order", "sbk_writer_open", "sbk_writer_particles",
     "sbk_writer_cloud", "sbk_writer_set_ion", "sbk_writer_x", "sbk_writer_close", "sbk_import", "sbk_init_cloud", "sbk_mass_table_open",
     "sbk_mass_table_lookup", "sbk_mass_table_close", "sbk_parse_species", "sbk_run_ini", "sbk_ini_describe", "sbk_ini_value", "sbk_ini_names",
@@ -180,8 +181,9 @@ def lib():
         "sbk_eval_force": (c_int, [vp, c_int, dp]),
         "sbk_eval_coulomb": (c_int, [vp, c_int, c_int, dp]),
         "sbk_set_launch": (c_int, [vp, c_int, c_int, c_int, c_int]),
+        "sbk_set_coulomb_kernel": (c_int, [vp, c_int]),
         "sbk_time_coulomb": (c_int, [vp, c_int, c_int, dp]),
-        "sbk_autotune": (c_int, [vp, c_int, ip, ip, ip, ip, ctypes.c_char_p]),
+        "sbk_autotune": (c_int, [vp, c_int, ip, ip, ip, ip, ip, ctypes.c_char_p]),
         "sbk_load_tuning": (c_int, [vp, c_int, ctypes.c_char_p]),
         "sbk_spatial_order": (c_int, [c_int, dp, ip]),
         "sbk_writer_open": (c_int, [ctypes.c_char_p, c_int, ctypes.POINTER(ctypes.c_char_p), dp, c_int,
@@ -485,6 +487,11 @@ class Engine:
         _check(lib().sbk_set_launch(self._h, int(targets_per_thread), int(source_groups),
                                     int(slices), int(local_coords)))
 
+    def set_coulomb_kernel(self, kernel):
+        """'auto', 'tiled' or 'stream' (see sbk_set_coulomb_kernel)."""
+        k = {"auto": -1, "tiled": 0, "stream": 1}[kernel] if isinstance(kernel, str) else int(kernel)
+        _check(lib().sbk_set_coulomb_kernel(self._h, k))
+
     def advance(self, duration, max_attempts=1 << 62):
         _check(lib().sbk_advance(self._h, float(duration), int(max_attempts)))
 
@@ -577,11 +584,21 @@ class Engine:
         return True
 
     def autotune(self, precision="mixed", table_path=None):
+        """Tries the launch shapes and Coulomb kernels and keeps the fastest:
+        (targets/thread, source groups, slices, local coords, kernel)."""
         p = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         t = ctypes.c_int(); q = ctypes.c_int(); s = ctypes.c_int(); lc = ctypes.c_int()
+        k = ctypes.c_int()
         _check(lib().sbk_autotune(self._h, p, ctypes.byref(t), ctypes.byref(q), ctypes.byref(s),
-                                  ctypes.byref(lc), table_path.encode() if table_path else None))
-        return t.value, q.value, s.value, lc.value
+                                  ctypes.byref(lc), ctypes.byref(k),
+                                  table_path.encode() if table_path else None))
+        return t.value, q.value, s.value, lc.value, k.value
+
+    def set_shape(self, shape):
+        """Applies (targets/thread, source groups, slices, local coords[, kernel])."""
+        self.set_launch(*shape[:4])
+        if len(shape) > 4:
+            self.set_coulomb_kernel(int(shape[4]))
 
 
 def nbody_accel(pos):

@@ -108,7 +108,7 @@ std::vector<sbk::CoulombItem> cunbody_items(int ni, int nj, int tpt, int q, int 
     int S = 1;
     if (precision != SBK_FP64_EXACT) {
         const long long capacity =
-            (long long)sbk::coulomb_blocks_per_cu(precision, tpt, unit, q, 0) * cus;
+            (long long)sbk::coulomb_blocks_per_cu(precision, tpt, unit, q, 0, sbk::KK_TILED) * cus;
         const long long target = 8 * capacity;
         const long long cap = std::max(1, nj / ((long long)ib * 8 < capacity ? 64 : 512));
         S = (int)std::max(1LL, std::min<long long>((target + ib / 2) / ib, cap));
@@ -348,7 +348,7 @@ int cunbody_force_static(double xj[][3], double mj[], double xi[][3], double eps
     const sbk::CoulombArgs a = cunbody_args(dxi, (long long)kMax, dxj, (long long)kMax, dm, dmf, cp,
                                             (long long)kMax, ditems, (int)items.size(), eps2,
                                             same_positions(xi, xj, ni, nj), unit, precision);
-    if (sbk::launch_coulomb(precision, tpt, q, a, nullptr)) {
+    if (sbk::launch_coulomb(precision, tpt, q, sbk::KK_TILED, a, nullptr)) {
         std::snprintf(msg, sizeof(msg), "sbk_cunbody_force_static: Coulomb kernel launch failed: %s",
                       hipGetErrorString(hipGetLastError()));
         return sbk::record_error(SBK_EHIP, msg);
@@ -431,7 +431,7 @@ void cunbody1_force_begin(double xj[][3], double mj[], double xi[][3], double ep
     const sbk::CoulombArgs a =
         cunbody_args(dx, ci, dj, cj, dm, g.d_mf, g.d_cp, stride, g.d_items, g.nitems, eps2,
                      same_positions(xi, xj, ni, nj), unit, g.precision);
-    if (sbk::launch_coulomb(g.precision, tpt, q, a, g.stream)) die("Coulomb kernel launch", hipGetLastError());
+    if (sbk::launch_coulomb(g.precision, tpt, q, sbk::KK_TILED, a, g.stream)) die("Coulomb kernel launch", hipGetLastError());
     hipLaunchKernelGGL(reduce_slices, dim3((ni + 255) / 256), dim3(256), 0, g.stream,
                        (const double *)g.d_cp, stride, S, ni, dai);
     CHK(hipGetLastError());

@@ -136,7 +136,8 @@ struct sbk_engine {
     int groups = 4;
     int slices_req = 0;
     int local_req = -1;          // -1: automatic
-    std::map<std::tuple<int, int, int, int, int, int>, ItemList> lists;
+    int kernel_req = -1;         // SBK_KERNEL_*; -1: automatic
+    std::map<std::tuple<int, int, int, int, int, int, int>, ItemList> lists;
 
     // graph of one attempt
     hipGraph_t graph = nullptr;
@@ -339,6 +340,19 @@ int targets_per_thread(const sbk_engine *e, int precision)
     return n2 >= 2.5e8 ? 2 : 1;
 }
 
+// Coulomb kernel of a launch: the streamed kernel exists for the mixed
+// precisions with 4 source groups and 1, 2 or 4 targets per thread.
+int coulomb_kernel(const sbk_engine *e, int precision)
+{
+    if (precision != SBK_MIXED && precision != SBK_MIXED_KAHAN) return sbk::KK_TILED;
+    if (e->groups != 4) return sbk::KK_TILED;
+    const int tpt = targets_per_thread(e, precision);
+    if (tpt != 1 && tpt != 2 && tpt != 4) return sbk::KK_TILED;
+    // automatic: the streamed kernel, measured on MI355X as fast or faster than the
+    // tiled one on every benchmark workload (65,536 ions -0.4%, 262,144 ions -0.3%)
+    return e->kernel_req == SBK_KERNEL_TILED ? sbk::KK_TILED : sbk::KK_STREAM;
+}
+
 // Builds (or returns the cached) Coulomb work list for a precision and
 // launch shape.  `only_coulomb` keeps the clouds whose Coulomb switch is on.
 int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
@@ -347,8 +361,9 @@ int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
     const int tpt = exact ? 1 : targets_per_thread(e, precision);
     const int q = exact ? 1 : e->groups;
     const int sreq = exact ? 1 : e->slices_req;
+    const int kern = coulomb_kernel(e, precision);
     auto key = std::make_tuple(precision, tpt, q, sreq, only_coulomb ? 1 : 0,
-                               local_coords(e, precision));
+                               local_coords(e, precision), kern);
     auto it = e->lists.find(key);
     if (it != e->lists.end()) {
         *out = &it->second;
@@ -357,7 +372,7 @@ int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
     const int blk = sbk::kBlock / q * tpt;   // targets per work-group
     const long long capacity =
         (long long)sbk::coulomb_blocks_per_cu(precision, tpt, e->unit_charges ? 1 : 0, q,
-                                              local_coords(e, precision)) *
+                                              local_coords(e, precision), kern) *
         e->cus;
     std::vector<sbk::PlanCloud> pc(e->clouds.size());
     for (size_t ci = 0; ci < e->clouds.size(); ++ci) {
@@ -711,7 +726,8 @@ int launch_coulomb_timed(sbk_engine *e, int precision, const sbk::CoulombArgs &a
     if (precision == SBK_FP64 || precision == SBK_FP64_RSQ)
         b.eps_d = 1e-40;   // the guard of bin/force.cpp:136
     b.local_coords = local_coords(e, precision);
-    if (sbk::launch_coulomb(kp(precision), targets_per_thread(e, precision), e->groups, b, e->stream))
+    if (sbk::launch_coulomb(kp(precision), targets_per_thread(e, precision), e->groups,
+                            coulomb_kernel(e, precision), b, e->stream))
         return fail(SBK_EHIP, "Coulomb kernel launch failed: %s",
                     hipGetErrorString(hipGetLastError()));
     if (evp) HIP_TRY(hipEventRecord(evp->second, e->stream));
@@ -1397,6 +1413,18 @@ int sbk_set_launch(sbk_engine *e, int tpt, int groups, int slices, int local)
     return 0;
 }
 
+int sbk_set_coulomb_kernel(sbk_engine *e, int kernel)
+{
+    if (!e) return fail(SBK_EINVAL, "null engine");
+    if (kernel != SBK_KERNEL_AUTO && kernel != SBK_KERNEL_TILED && kernel != SBK_KERNEL_STREAM)
+        return fail(SBK_EINVAL, "unknown Coulomb kernel %d", kernel);
+    if (kernel != e->kernel_req) {
+        e->kernel_req = kernel;
+        destroy_graph(e);
+    }
+    return 0;
+}
+
 // Mean time of `reps` Coulomb launches on the current positions; with_sum
 // adds the pass that reads every slice partial back (what each stage kernel
 // pays), so launch shapes with many slices are charged for their partials.
@@ -1416,7 +1444,8 @@ static int time_coulomb_pass(sbk_engine *e, int precision, int reps, bool with_s
     HIP_TRY(hipEventCreate(&t1));
     HIP_TRY(hipEventRecord(t0, e->stream));
     for (int i = 0; i < reps; ++i) {
-        if (sbk::launch_coulomb(kp(precision), targets_per_thread(e, precision), e->groups, a, e->stream))
+        if (sbk::launch_coulomb(kp(precision), targets_per_thread(e, precision), e->groups,
+                                coulomb_kernel(e, precision), a, e->stream))
             return fail(SBK_EHIP, "Coulomb kernel launch failed");
         if (with_sum && sbk::launch_eval_derivs(d, 0, e->stream))
             return fail(SBK_EHIP, "partial-sum kernel launch failed");
@@ -1442,7 +1471,7 @@ int sbk_time_coulomb(sbk_engine *e, int precision, int reps, double *ms_per_laun
 }
 
 int sbk_autotune(sbk_engine *e, int precision, int *tpt_out, int *groups_out, int *slices_out,
-                 int *local_out, const char *table)
+                 int *local_out, int *kernel_out, const char *table)
 {
     int rc = prepare(e);
     if (rc) return rc;
@@ -1451,18 +1480,22 @@ int sbk_autotune(sbk_engine *e, int precision, int *tpt_out, int *groups_out, in
         if (groups_out) *groups_out = 1;
         if (slices_out) *slices_out = 1;
         if (local_out) *local_out = 0;
+        if (kernel_out) *kernel_out = SBK_KERNEL_TILED;
         return 0;
     }
     const int tpts[3] = {1, 2, 4};
     const int qs[2] = {1, 4};
     const int rounds[6] = {0, 1, 2, 3, 4, 8};   // 0: automatic split
     const bool may_local = e->spatial && (precision == SBK_MIXED || precision == SBK_MIXED_KAHAN);
-    int best_t = e->tpt, best_q = e->groups, best_s = e->slices_req, best_l = e->local_req;
+    const bool may_stream = precision == SBK_MIXED || precision == SBK_MIXED_KAHAN;
+    int best_t = e->tpt, best_q = e->groups, best_s = e->slices_req, best_l = e->local_req,
+        best_k = e->kernel_req;
     double best = 1e300;
     char dev_name[128] = {0};
     sbk_device_info(dev_name, sizeof(dev_name), nullptr, nullptr);
     FILE *fp = table ? std::fopen(table, "a") : nullptr;
-    const int old_t = e->tpt, old_q = e->groups, old_s = e->slices_req, old_l = e->local_req;
+    const int old_t = e->tpt, old_q = e->groups, old_s = e->slices_req, old_l = e->local_req,
+              old_k = e->kernel_req;
     {
         // about 50 ms of launches first: the first shape timed on an idle
         // device otherwise reads several per cent slow while its clocks rise
@@ -1473,17 +1506,22 @@ int sbk_autotune(sbk_engine *e, int precision, int *tpt_out, int *groups_out, in
             rc = time_coulomb_pass(e, precision, warm, true, &ms0);
         }
     }
+    for (int kk = SBK_KERNEL_TILED; kk <= (may_stream ? SBK_KERNEL_STREAM : SBK_KERNEL_TILED) && !rc; ++kk)
     for (int l = 0; l <= (may_local ? 1 : 0) && !rc; ++l) {
         e->local_req = l;
+        e->kernel_req = kk;
         for (int t : tpts) {
             for (int q : qs) {
+                if (kk == SBK_KERNEL_STREAM && q != 4) continue;   // four streams per work-group
                 long long blocks = 0;
                 const int blk = sbk::kBlock / q * t;
                 for (auto &c : e->clouds)
                     if (c.n > 0) blocks += (c.n + blk - 1) / blk;
                 const long long capacity =
                     (long long)sbk::coulomb_blocks_per_cu(precision, t, e->unit_charges ? 1 : 0, q,
-                                                          local_coords(e, precision)) *
+                                                          local_coords(e, precision),
+                                                          kk == SBK_KERNEL_STREAM ? sbk::KK_STREAM
+                                                                                  : sbk::KK_TILED) *
                     e->cus;
                 for (int k : rounds) {
                     const int slices = k == 0 ? 0
@@ -1498,15 +1536,16 @@ int sbk_autotune(sbk_engine *e, int precision, int *tpt_out, int *groups_out, in
                     rc = time_coulomb_pass(e, precision, 3, true, &ms);
                     if (rc) break;
                     if (fp)
-                        std::fprintf(fp, "%s\t%d\t%zu\t%d\t%d\t%d\t%d\t%d\t%.6f\n", dev_name,
+                        std::fprintf(fp, "%s\t%d\t%zu\t%d\t%d\t%d\t%d\t%d\t%d\t%.6f\n", dev_name,
                                      e->ntot, e->clouds.size(), precision, t, q, slices,
-                                     local_coords(e, precision), ms);
+                                     local_coords(e, precision), coulomb_kernel(e, precision), ms);
                     if (ms < best) {
                         best = ms;
                         best_t = t;
                         best_q = q;
                         best_s = slices;
                         best_l = l;
+                        best_k = kk;
                     }
                 }
                 if (rc) break;
@@ -1519,13 +1558,16 @@ int sbk_autotune(sbk_engine *e, int precision, int *tpt_out, int *groups_out, in
     e->groups = old_q;
     e->slices_req = old_s;
     e->local_req = old_l;
+    e->kernel_req = old_k;
     if (rc) return rc;
     rc = sbk_set_launch(e, best_t, best_q, best_s, best_l);
+    if (!rc) rc = sbk_set_coulomb_kernel(e, best_k);
     if (rc) return rc;
     if (tpt_out) *tpt_out = best_t;
     if (groups_out) *groups_out = best_q;
     if (slices_out) *slices_out = best_s;
     if (local_out) *local_out = local_coords(e, precision);
+    if (kernel_out) *kernel_out = coulomb_kernel(e, precision);
     return 0;
 }
 
@@ -1541,7 +1583,7 @@ int sbk_load_tuning(sbk_engine *e, int precision, const char *table)
     char line[512], dev[256];
     int ions, clouds, prec, t, q, sl, lc;
     double ms, best = 1e300;
-    int bt = 0, bq = 0, bs = 0, bl = 0;
+    int bt = 0, bq = 0, bs = 0, bl = 0, bk = SBK_KERNEL_TILED;
     while (std::fgets(line, sizeof(line), fp)) {
         char *rest = nullptr;
         int used = 0;
@@ -1549,9 +1591,14 @@ int sbk_load_tuning(sbk_engine *e, int precision, const char *table)
                         &sl, &used) != 7)
             continue;
         rest = line + used;
-        double f1 = 0.0, f2 = 0.0;
-        const int nf = std::sscanf(rest, "%lf %lf", &f1, &f2);
-        if (nf == 2) {            // ... local_coords ms
+        double f1 = 0.0, f2 = 0.0, f3 = 0.0;
+        int kern = SBK_KERNEL_TILED;
+        const int nf = std::sscanf(rest, "%lf %lf %lf", &f1, &f2, &f3);
+        if (nf == 3) {            // ... local_coords kernel ms
+            lc = (int)f1;
+            kern = (int)f2;
+            ms = f3;
+        } else if (nf == 2) {     // older lines without the kernel column (tiled)
             lc = (int)f1;
             ms = f2;
         } else if (nf == 1) {     // older lines without the local_coords column
@@ -1569,11 +1616,13 @@ int sbk_load_tuning(sbk_engine *e, int precision, const char *table)
             bq = q;
             bs = sl;
             bl = lc;
+            bk = kern;
         }
     }
     std::fclose(fp);
     if (best == 1e300) return fail(SBK_ESTATE, "no matching line in %s", table);
-    return sbk_set_launch(e, bt, bq, bs, bl);
+    rc = sbk_set_launch(e, bt, bq, bs, bl);
+    return rc ? rc : sbk_set_coulomb_kernel(e, bk);
 }
 
 }  // extern "C"

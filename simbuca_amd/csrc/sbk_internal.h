@@ -141,12 +141,18 @@ enum KernelPrecision {
     KP_FP64_EXACT = 0, KP_FP64 = 1, KP_MIXED = 2, KP_MIXED_KAHAN = 3, KP_FP64_RSQ = 4
 };
 
+// Coulomb kernels: KK_TILED (coulomb_tiled / coulomb_fp64_exact: the work-group's
+// wave groups share each LDS source tile) or KK_STREAM (coulomb_stream, mixed
+// modes, q = 4: each wave streams its own sub-tiles with no barrier inside the
+// source loop).  Both take the same work items.
+enum CoulombKernel { KK_TILED = 0, KK_STREAM = 1 };
+
 // q: wave groups of a work-group that split the sources of one target set (1 or 4)
-int launch_coulomb(int precision, int targets_per_thread, int q, const CoulombArgs &a,
+int launch_coulomb(int precision, int targets_per_thread, int q, int kernel, const CoulombArgs &a,
                    void *stream);
 // resident work-groups per CU of the Coulomb kernel a launch would use
 int coulomb_blocks_per_cu(int precision, int targets_per_thread, int unit_weights, int q,
-                          int local_coords);
+                          int local_coords, int kernel);
 int launch_stage(int stage, const EngineDev &d, void *stream);
 int launch_controller(const EngineDev &d, void *stream);
 int launch_commit(const EngineDev &d, void *stream);

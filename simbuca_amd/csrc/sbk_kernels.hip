@@ -412,6 +412,255 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// Coulomb direct sum, streamed (mixed modes).  Same work items, targets,
+// pair arithmetic, local coordinates and accumulation as coulomb_tiled with
+// Q = 4, but the four waves of a work-group never wait for one another
+// inside the source loop: every wave holds all 64 R targets of the item
+// (target r*64 + lane), computes the same target box by itself, and streams
+// its own sub-tiles (wave w takes sub-tiles w, w+4, w+8, ... of the slice)
+// through a private LDS region, loading the next sub-tile into registers
+// while it sums the current one.  The only barrier is the fixed-order
+// combination of the four waves' sums at the end.  A wave that has more
+// near sources therefore no longer holds the others back at every tile, and
+// the global-load latency of the next sub-tile overlaps the sums.
+// ---------------------------------------------------------------------------
+template <int R, int MODE, bool UNIT, bool LOC>
+__global__ __launch_bounds__(kBlock) void coulomb_stream(CoulombArgs a)
+{
+    static_assert(MODE == KP_MIXED || MODE == KP_MIXED_KAHAN, "a mixed-precision kernel");
+    constexpr int NW = kBlock / 64;      // waves per work-group: 4 source streams
+    constexpr int W = 64;                // sources per sub-tile = lanes per wave
+    __shared__ double lx[NW * W], ly[NW * W], lz[NW * W];   // FP64 sources (near path); reused
+    __shared__ float4 lf[NW * W];                           // (offset or 0, weight)
+
+    const CoulombItem it = a.items[xcd_item(blockIdx.x, gridDim.x)];
+    // uniform over the work-group (one item), so either every wave reaches
+    // the closing barrier or none does
+    if (a.state && (!a.state[it.cloud].active || (a.skip_valid_k1 && a.state[it.cloud].k1_valid)))
+        return;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    double *const wx = lx + wave * W;
+    double *const wy = ly + wave * W;
+    double *const wz = lz + wave * W;
+    float4 *const wf = lf + wave * W;
+
+    double px[R], py[R], pz[R];
+    double Ax[R], Ay[R], Az[R];
+    float Sx[R], Sy[R], Sz[R], Cx[R], Cy[R], Cz[R];
+    float fx[R], fy[R], fz[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int g = it.i0 + r * W + lane;
+        const bool v = g < it.i1;
+        px[r] = v ? a.tx[g] : 0.0;
+        py[r] = v ? a.ty[g] : 0.0;
+        pz[r] = v ? a.tz[g] : 0.0;
+        Ax[r] = Ay[r] = Az[r] = 0.0;
+        Sx[r] = Sy[r] = Sz[r] = 0.f;
+        Cx[r] = Cy[r] = Cz[r] = 0.f;
+        fx[r] = fy[r] = fz[r] = 0.f;
+    }
+    const float epsf = a.eps_f;
+
+    // LOC: origin = centre of the targets' bounding box, reduced inside the
+    // wave (each wave holds every target, so all four find the same box)
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    float far2 = 0.f;
+    if (LOC) {
+        double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (it.i0 + r * W + lane < it.i1) {
+                mn[0] = fmin(mn[0], px[r]); mx[0] = fmax(mx[0], px[r]);
+                mn[1] = fmin(mn[1], py[r]); mx[1] = fmax(mx[1], py[r]);
+                mn[2] = fmin(mn[2], pz[r]); mx[2] = fmax(mx[2], pz[r]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = wave_min(mn[k]);
+            mx[k] = wave_max(mx[k]);
+        }
+        ox = 0.5 * (mn[0] + mx[0]);
+        oy = 0.5 * (mn[1] + mx[1]);
+        oz = 0.5 * (mn[2] + mx[2]);
+        const float hx = (float)(0.5 * (mx[0] - mn[0]));
+        const float hy = (float)(0.5 * (mx[1] - mn[1]));
+        const float hz = (float)(0.5 * (mx[2] - mn[2]));
+        far2 = (1.f + kNearFactor) * (1.f + kNearFactor) * (hx * hx + hy * hy + hz * hz);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            fx[r] = (float)(px[r] - ox);
+            fy[r] = (float)(py[r] - oy);
+            fz[r] = (float)(pz[r] - oz);
+        }
+    }
+
+    // this wave's first sub-tile, loaded ahead
+    int base = it.j0 + wave * W;
+    double nx = 0.0, ny = 0.0, nz = 0.0;
+    float nq = 1.f;
+    if (base + lane < it.j1) {
+        nx = a.sx[base + lane];
+        ny = a.sy[base + lane];
+        nz = a.sz[base + lane];
+        if (!UNIT) nq = a.sqf[base + lane];
+    }
+    for (; base < it.j1; base += NW * W) {
+        const int m = min(W, it.j1 - base);
+        const double sx = nx, sy = ny, sz = nz;
+        const float sq = nq;
+        {
+            const int jn = base + NW * W + lane;
+            if (jn < it.j1) {
+                nx = a.sx[jn];
+                ny = a.sy[jn];
+                nz = a.sz[jn];
+                if (!UNIT) nq = a.sqf[jn];
+            }
+        }
+        // every lane's reads of the previous sub-tile precede these writes:
+        // LDS instructions of one wave complete in order, and the fence keeps
+        // the compiler from moving the writes above the reads
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const bool valid = lane < m;
+        int nfar = 0;
+        if (LOC) {
+            // far sources first: a far source at its rank among the far lanes
+            // below it, a near one after all far ones (valid lanes are a prefix)
+            float4 s4 = make_float4(0.f, 0.f, 0.f, 1.f);
+            bool far_src = false;
+            if (valid) {
+                s4 = make_float4((float)(sx - ox), (float)(sy - oy), (float)(sz - oz), UNIT ? 1.f : sq);
+                far_src = fmaf(s4.x, s4.x, fmaf(s4.y, s4.y, s4.z * s4.z)) >= far2;
+            }
+            const unsigned long long far_mask = __ballot(far_src);
+            nfar = __popcll(far_mask);
+            const int far_below = (int)__builtin_amdgcn_mbcnt_hi(
+                (unsigned)(far_mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)far_mask, 0u));
+            const int slot = far_src ? far_below : nfar + (lane - far_below);
+            if (valid) {
+                wx[slot] = sx;
+                wy[slot] = sy;
+                wz[slot] = sz;
+                wf[slot] = s4;
+            }
+        } else if (valid) {
+            wx[lane] = sx;
+            wy[lane] = sy;
+            wz[lane] = sz;
+            if (!UNIT) wf[lane] = make_float4(0.f, 0.f, 0.f, sq);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+        float ax[R], ay[R], az[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) ax[r] = ay[r] = az[r] = 0.f;
+        auto body = [&](int j, auto far_tag) {
+            constexpr bool FAR = decltype(far_tag)::value;
+            double ssx = 0.0, ssy = 0.0, ssz = 0.0;
+            float4 s = make_float4(0.f, 0.f, 0.f, 1.f);
+            if (FAR) {
+                s = wf[j];
+            } else {
+                ssx = wx[j]; ssy = wy[j]; ssz = wz[j];
+                if (!UNIT) s.w = wf[j].w;
+            }
+            const float q = s.w;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const float dx = FAR ? s.x - fx[r] : (float)(ssx - px[r]);
+                const float dy = FAR ? s.y - fy[r] : (float)(ssy - py[r]);
+                const float dz = FAR ? s.z - fz[r] : (float)(ssz - pz[r]);
+                float d2 = fmaf(dz, dz, epsf);
+                d2 = fmaf(dy, dy, d2);
+                d2 = fmaf(dx, dx, d2);
+                const float inv = __builtin_amdgcn_rsqf(d2);
+                const float w = UNIT ? inv * (inv * inv) : (q * inv) * (inv * inv);
+                ax[r] = fmaf(dx, w, ax[r]);
+                ay[r] = fmaf(dy, w, ay[r]);
+                az[r] = fmaf(dz, w, az[r]);
+            }
+        };
+        auto run = [&](int j0, int j1, auto far_tag) {
+            constexpr int U = 8;
+            if (j1 - j0 == W) {
+#pragma unroll U
+                for (int j = 0; j < W; ++j) body(j0 + j, far_tag);
+            } else {
+                int j = j0;
+                for (; j + U <= j1; j += U) {
+#pragma unroll
+                    for (int k = 0; k < U; ++k) body(j + k, far_tag);
+                }
+                for (; j < j1; ++j) body(j, far_tag);
+            }
+        };
+        if constexpr (LOC) {
+            const int nf = __builtin_amdgcn_readfirstlane(nfar);
+            if (nf > 0) run(0, nf, std::true_type{});
+            if (nf < m) run(nf, m, std::false_type{});
+        } else {
+            run(0, m, std::false_type{});
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (MODE == KP_MIXED_KAHAN) {
+                float y, t;
+                y = ax[r] - Cx[r]; t = Sx[r] + y; Cx[r] = (t - Sx[r]) - y; Sx[r] = t;
+                y = ay[r] - Cy[r]; t = Sy[r] + y; Cy[r] = (t - Sy[r]) - y; Sy[r] = t;
+                y = az[r] - Cz[r]; t = Sz[r] + y; Cz[r] = (t - Sz[r]) - y; Sz[r] = t;
+            } else {
+                Ax[r] += (double)ax[r];
+                Ay[r] += (double)ay[r];
+                Az[r] += (double)az[r];
+            }
+        }
+    }
+
+    if (MODE == KP_MIXED_KAHAN) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            Ax[r] = (double)Sx[r] - (double)Cx[r];
+            Ay[r] = (double)Sy[r] - (double)Cy[r];
+            Az[r] = (double)Sz[r] - (double)Cz[r];
+        }
+    }
+    // the four streams' sums, wave 0 first, one target set at a time
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        __syncthreads();
+        lx[tid] = Ax[r];
+        ly[tid] = Ay[r];
+        lz[tid] = Az[r];
+        __syncthreads();
+        if (wave == 0) {
+            for (int q = 1; q < NW; ++q) {
+                Ax[r] += lx[q * W + lane];
+                Ay[r] += ly[q * W + lane];
+                Az[r] += lz[q * W + lane];
+            }
+        }
+    }
+    if (wave != 0) return;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int g = it.i0 + r * W + lane;
+        if (g < it.i1) {
+            const double k = a.kc ? a.kc[g] : 1.0;
+            double *o = a.out + (long long)it.slice * 3 * a.stride + g;
+            o[0] = k * Ax[r];
+            o[a.stride] = k * Ay[r];
+            o[2 * a.stride] = k * Az[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Dormand-Prince 5(4) tableau, bin/ode.cpp:76-86.  kA[s][j] is the
 // coefficient of k_j in the input of stage s (kA[7] = b), kE the error
 // weights e = b - b*.  Zero entries are skipped, as the reference's
@@ -859,8 +1108,32 @@ static const void *tiled_fn(int precision, int unit, int q, int loc)
     return nullptr;
 }
 
-static const void *coulomb_fn(int precision, int tpt, int unit, int q, int loc)
+template <int R>
+static const void *stream_fn(int precision, int unit, int loc)
 {
+#define SBK_STREAM(M, U, L) reinterpret_cast<const void *>(&coulomb_stream<R, M, U, L>)
+    if (precision == KP_MIXED)
+        return unit ? (loc ? SBK_STREAM(KP_MIXED, true, true) : SBK_STREAM(KP_MIXED, true, false))
+                    : (loc ? SBK_STREAM(KP_MIXED, false, true) : SBK_STREAM(KP_MIXED, false, false));
+    if (precision == KP_MIXED_KAHAN)
+        return unit ? (loc ? SBK_STREAM(KP_MIXED_KAHAN, true, true) : SBK_STREAM(KP_MIXED_KAHAN, true, false))
+                    : (loc ? SBK_STREAM(KP_MIXED_KAHAN, false, true) : SBK_STREAM(KP_MIXED_KAHAN, false, false));
+#undef SBK_STREAM
+    return nullptr;
+}
+
+static const void *coulomb_fn(int precision, int tpt, int unit, int q, int loc, int kernel)
+{
+    if (kernel == KK_STREAM) {
+        // four source streams per work-group over one target set of 64 tpt
+        if (q != 4) return nullptr;
+        switch (tpt) {
+        case 1: return stream_fn<1>(precision, unit, loc);
+        case 2: return stream_fn<2>(precision, unit, loc);
+        case 4: return stream_fn<4>(precision, unit, loc);
+        default: return nullptr;
+        }
+    }
     if (precision == KP_FP64_EXACT)
         return loc ? nullptr : reinterpret_cast<const void *>(&coulomb_fp64_exact);
     switch (tpt) {
@@ -872,9 +1145,9 @@ static const void *coulomb_fn(int precision, int tpt, int unit, int q, int loc)
     }
 }
 
-int coulomb_blocks_per_cu(int precision, int tpt, int unit, int q, int loc)
+int coulomb_blocks_per_cu(int precision, int tpt, int unit, int q, int loc, int kernel)
 {
-    const void *f = coulomb_fn(precision, tpt, unit, q, loc);
+    const void *f = coulomb_fn(precision, tpt, unit, q, loc, kernel);
     if (!f) return 1;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kBlock, 0) != hipSuccess || nb < 1)
@@ -882,10 +1155,10 @@ int coulomb_blocks_per_cu(int precision, int tpt, int unit, int q, int loc)
     return nb;
 }
 
-int launch_coulomb(int precision, int tpt, int q, const CoulombArgs &a, void *stream)
+int launch_coulomb(int precision, int tpt, int q, int kernel, const CoulombArgs &a, void *stream)
 {
     if (a.nitems <= 0) return 0;
-    const void *f = coulomb_fn(precision, tpt, a.unit_weights, q, a.local_coords);
+    const void *f = coulomb_fn(precision, tpt, a.unit_weights, q, a.local_coords, kernel);
     if (!f) return -1;
     void *args[] = {const_cast<CoulombArgs *>(&a)};
     if (hipLaunchKernel(f, dim3(a.nitems), dim3(kBlock), args, 0, (hipStream_t)stream) !=

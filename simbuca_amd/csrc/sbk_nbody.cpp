@@ -80,7 +80,7 @@ void build_items(int n)
     const int blk = sbk::kBlock / kGroups * kTpt;
     const int ib = (n + blk - 1) / blk;
     const long long capacity =
-        (long long)sbk::coulomb_blocks_per_cu(sbk::KP_MIXED, kTpt, 1, kGroups, 0) * g.cus;
+        (long long)sbk::coulomb_blocks_per_cu(sbk::KP_MIXED, kTpt, 1, kGroups, 0, sbk::KK_TILED) * g.cus;
     int S = (int)std::max(1LL, std::min<long long>((8 * capacity + ib / 2) / ib, std::max(1, n / 512)));
     // slices of whole LDS tiles for all wave groups
     const int quantum = sbk::kSubTile * kGroups;
@@ -189,7 +189,7 @@ void simulation_Async(sbk_float4 *h_p, sbk_float3 *res, int nbody, int p, int q,
     a.state = nullptr;
     a.eps_f = 1e-7f;             // softeningSquared, bin/nbody_kernel.cu:19
     a.eps_d = 1e-7;
-    if (sbk::launch_coulomb(sbk::KP_MIXED, kTpt, kGroups, a, nullptr))
+    if (sbk::launch_coulomb(sbk::KP_MIXED, kTpt, kGroups, sbk::KK_TILED, a, nullptr))
         die("Coulomb kernel launch", hipGetLastError());
     hipLaunchKernelGGL(gather_slices, dim3((nbody + 255) / 256), dim3(256), 0, nullptr,
                        (const double *)g.d_cp, g.n_max, g.slices, nbody, g.d_out);

@@ -40,10 +40,26 @@ Plan plan_coulomb(const std::vector<PlanCloud> &clouds, int blk, int quantum, in
         int chunk = (c.n + S - 1) / S;
         if (!exact) chunk = ((chunk + quantum - 1) / quantum) * quantum;
         S = (c.n + chunk - 1) / chunk;
+        // slice boundaries (relative to the cloud's first ion)
+        std::vector<int> cut;
+        for (int s = 0; s <= S; ++s) cut.push_back(std::min(c.n, s * chunk));
+#ifdef SBK_TAPER_F
+        if (!exact && sreq <= 0 && S >= 8 && chunk % (2 * quantum) == 0) {
+            // the first part of the sources in double-length slices, the last
+            // SBK_TAPER_F of them in half-length ones
+            const int tail = ((int)(SBK_TAPER_F * c.n) / chunk) * chunk;
+            cut.clear();
+            int j = 0;
+            cut.push_back(0);
+            while (j < c.n - tail) { j = std::min(c.n - tail, j + 2 * chunk); cut.push_back(j); }
+            while (j < c.n) { j = std::min(c.n, j + chunk / 2); cut.push_back(j); }
+            S = (int)cut.size() - 1;
+        }
+#endif
         L.slices[ci] = S;
         L.max_slices = std::max(L.max_slices, S);
         for (int s = 0; s < S; ++s) {
-            const int j0 = c.off + s * chunk, j1 = c.off + std::min(c.n, (s + 1) * chunk);
+            const int j0 = c.off + cut[s], j1 = c.off + cut[s + 1];
             for (int b = 0; b < ib; ++b) {
                 CoulombItem w{};
                 w.cloud = (int)ci;

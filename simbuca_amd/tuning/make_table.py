@@ -4,7 +4,7 @@ The reference ships per-GPU launch tables read at start-up
 (libraries/NBODY/<GPU>_NB0DY.dat, read by bin/nbody.cpp:275-321).  This is
 their counterpart: sbk_autotune times every launch shape on the benchmark
 workloads, and the lines ("device ions clouds precision targets/thread
-source-groups slices local-coords ms") are written to
+source-groups slices local-coords kernel ms") are written to
 simbuca_amd/tuning/<arch>.tsv.
 The engine applies the fastest line that matches its device, ion count,
 cloud count and precision with sbk_load_tuning.
@@ -28,7 +28,8 @@ def main():
         os.remove(out)
     for k in (1, 2, 3, 4, 5):
         w = synth.workload(k)
-        for precision in sorted({"fp64", "mixed_kahan" if w.precision == "mixed_kahan" else "mixed"}):
+        for precision in sorted({"fp64", "fp64_rsq",
+                                 "mixed_kahan" if w.precision == "mixed_kahan" else "mixed"}):
             e = S.Engine(precision=precision)
             for c in w.clouds:
                 e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)

@@ -351,6 +351,58 @@ def test_fp64_rsq_against_the_oracle(rng, n):
     assert st["error"] == 0
 
 
+@pytest.mark.parametrize("precision", ["mixed", "mixed_kahan"])
+@pytest.mark.parametrize("unit", [True, False])
+def test_stream_kernel_equals_tiled_kernel(rng, precision, unit):
+    """The streamed kernel gives wave w the same sub-tiles the tiled kernel's
+    wave group w takes (w, w+4, ... of each slice), stages them in the same
+    order, folds them in the same order and combines the four waves in the
+    same order: its sums are the tiled kernel's bit for bit, for every shape
+    it supports, and within 1e-6 of the FP64-exact sum.  Clouds of uneven
+    sizes (ragged last tiles, a 3-ion cloud) share the launch."""
+    sizes = [3001, 64, 3, 700]
+    sf, _ = _both_fields(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0)
+    clouds = []
+    for n in sizes:
+        amu, m, ch, pos, vel = _rand_cloud(rng, n, sigma=1e-3)
+        if unit:
+            ch = np.ones(n, np.int32)
+        clouds.append((pos, vel, m, ch))
+
+    def engine():
+        e = S.Engine(precision=precision, time_mode=S.TIME_STAGE, rtol=1e-9, atol=1e-9)
+        for pos, vel, m, ch in clouds:
+            e.add_cloud(pos, vel, m, ch, sf)
+        return e
+
+    with engine() as e:
+        exact = [e.eval_coulomb(c, "fp64_exact") for c in range(len(sizes))]
+        for tpt in (1, 2, 4):
+            for slices in (0, 1, 3):
+                for local in (0, 1):
+                    print(f"stream vs tiled {precision} unit={unit} tpt={tpt} slices={slices} "
+                          f"local={local}", flush=True)
+                    e.set_launch(tpt, 4, slices, local)
+                    e.set_coulomb_kernel("tiled")
+                    tiled = [e.eval_coulomb(c, precision) for c in range(len(sizes))]
+                    e.set_coulomb_kernel("stream")
+                    stream = [e.eval_coulomb(c, precision) for c in range(len(sizes))]
+                    for c in range(len(sizes)):
+                        assert np.array_equal(stream[c], tiled[c]), (tpt, slices, local, c)
+                        assert _rows_error(stream[c], exact[c]) < 1e-6
+    # whole adaptive attempts replayed from the graph: the same states bit for bit
+    states = {}
+    for kern in ("tiled", "stream"):
+        with engine() as e:
+            e.set_launch(2, 4, 0, 1)
+            e.set_coulomb_kernel(kern)
+            e.attempts(3)
+            e.synchronize()
+            states[kern] = [e.state(c) for c in range(len(sizes))]
+    for (pt, vt), (ps, vs) in zip(states["tiled"], states["stream"]):
+        assert np.array_equal(pt, ps) and np.array_equal(vt, vs)
+
+
 def test_autotune_table_round_trip(rng, tmp_path):
     amu, m, ch, pos, vel = _rand_cloud(rng, 2000, sigma=1e-3)
     sf, _ = _both_fields(trap=0, coulomb=1)
@@ -360,8 +412,11 @@ def test_autotune_table_round_trip(rng, tmp_path):
         shape = e.autotune("mixed", table)
         ref = e.eval_coulomb(0, "mixed")
     lines = open(table).read().strip().splitlines()
-    assert len(lines) == 2 * 3 * 2 * 6                 # local coords x R x Q x slice rounds
+    # tiled: local coords x R x Q x slice rounds; streamed: local coords x R (Q = 4) x rounds
+    assert len(lines) == 2 * 3 * 2 * 6 + 2 * 3 * 6
     assert {line.split()[7] for line in lines} == {"0", "1"}
+    assert {line.split()[8] for line in lines} == {"0", "1"}   # both kernels
+    assert len(shape) == 5 and shape[4] in (0, 1)
     with S.Engine(precision="mixed") as e:
         e.add_cloud(pos, vel, m, ch, sf)
         assert e.load_tuning(table, "mixed")
