@@ -10,6 +10,9 @@
  *     in place of libcunbody1.a makes the reference's force() run its
  *     direct sum on the MI355X.
  *
+ *  1b. The reference's NBODY interface: include/simbuca_nbody.h, in
+ *     libsimbuca_nbody.so.
+ *
  *  2. sbk_*: a device-resident, step-level engine.  It replaces the
  *     reference's Dormand_Prince_5 + error_succes + step() commit
  *     (bin/ode.cpp:61-218,426-494) and the force() it calls
@@ -83,31 +86,9 @@ int sbk_cunbody_set_precision(int precision);
 int sbk_cunbody_force_static(double xj[][3], double mj[], double xi[][3], double eps2,
                              double ai[][3], int ni, int nj);
 
-/* ------------------------------------------------------------------------
- * Export set 1b -- reference NBODY interface, as its caller declares it
- * (bin/nbody.cpp:12-18; the reference's own kernel file defines different,
- * incompatible signatures at bin/nbody_kernel.cu:116,326).
- *
- * Positions go in as FP32 {x, y, z, w} (w ignored), accelerations come back
- * as FP32 {x, y, z}: acc_i = sum_j (x_j - x_i) (|x_j - x_i|^2 + 1e-7)^{-3/2}
- * (bin/nbody_kernel.cu:19,182-214; no charges, no ke).  The caller scales
- * by -coulomb_scale e^2 ke / m_i (bin/force.cpp:97-104).  Work is enqueued
- * on stream 0, so an event the caller records on stream 0 after
- * simulation_Async completes with it (bin/nbody.cpp:243-245,265-268).
- * p, q, ntiles are the reference's launch-table hints and are ignored.
- * initialize() allocates for up to n_max bodies on the device current at
- * that call; later calls use those buffers with the same device current.
- * Errors print a message and end the process with EXIT_FAILURE.
- * ---------------------------------------------------------------------- */
-typedef struct sbk_float4 { float x, y, z, w; } sbk_float4;   /* layout of float4 */
-typedef struct sbk_float3 { float x, y, z; } sbk_float3;      /* layout of float3 */
-void  initialize(int n_max, bool withCharge);
-float finalize(void);
-void  simulation_Async(sbk_float4 *h_p, sbk_float3 *res, int nbody, int p, int q, int ntiles);
-void  simulation_Async_begin(sbk_float4 *h_p, sbk_float3 *res, int nbody, int p, int q,
-                             int ntiles);
-void  simulation_Async_end(sbk_float4 *h_p, sbk_float3 *res, int nbody, int p, int q,
-                           int ntiles);
+/* Export set 1b, the reference's NBODY interface (initialize, finalize,
+ * simulation_Async*), is in include/simbuca_nbody.h and its own library,
+ * libsimbuca_nbody.so, so that this library exports no generic names. */
 
 /* ------------------------------------------------------------------------
  * Export set 2 -- device-resident engine.

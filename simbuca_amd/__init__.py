@@ -21,6 +21,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsimbuca_hip.so")
+NBODY_LIB_PATH = os.path.join(HERE, "libsimbuca_nbody.so")
 
 FP64_EXACT, FP64, MIXED, MIXED_KAHAN, FP64_RSQ = 0, 1, 2, 3, 4
 PRECISIONS = {"fp64_exact": FP64_EXACT, "fp64": FP64, "mixed": MIXED, "mixed_kahan": MIXED_KAHAN,
@@ -121,7 +122,6 @@ _lib = None
 EXPORTS = [
     "cunbody1_force", "cunbody1_force_begin", "cunbody1_force_end", "sbk_cunbody_set_precision",
     "sbk_cunbody_force_static",
-    "initialize", "finalize", "simulation_Async", "simulation_Async_begin", "simulation_Async_end",
     "sbk_default_config", "sbk_default_fields", "sbk_create", "sbk_destroy", "sbk_add_cloud",
     "sbk_set_fields", "sbk_set_step", "sbk_advance", "sbk_advance_until", "sbk_step", "sbk_attempts",
     "sbk_synchronize", "sbk_reorder", "sbk_set_crossing_plane", "sbk_take_crossings", "sbk_get_state", "sbk_get_status", "sbk_get_stats", "sbk_reset_stats",
@@ -132,6 +132,10 @@ EXPORTS = [
     "sbk_mass_table_lookup", "sbk_mass_table_close", "sbk_parse_species", "sbk_run_ini", "sbk_ini_describe", "sbk_ini_value", "sbk_ini_names",
     "sbk_last_error", "sbk_device_info",
 ]
+
+# every symbol include/simbuca_nbody.h declares (libsimbuca_nbody.so)
+NBODY_EXPORTS = ["initialize", "finalize", "simulation_Async", "simulation_Async_begin",
+                 "simulation_Async_end"]
 
 
 def lib():
@@ -154,11 +158,6 @@ def lib():
         "cunbody1_force_end": (None, [dp, dp, dp, c_dbl, dp, c_int, c_int]),
         "sbk_cunbody_set_precision": (c_int, [c_int]),
         "sbk_cunbody_force_static": (c_int, [dp, dp, dp, c_dbl, dp, c_int, c_int]),
-        "initialize": (None, [c_int, ctypes.c_bool]),
-        "finalize": (ctypes.c_float, []),
-        "simulation_Async": (None, [vp, vp, c_int, c_int, c_int, c_int]),
-        "simulation_Async_begin": (None, [vp, vp, c_int, c_int, c_int, c_int]),
-        "simulation_Async_end": (None, [vp, vp, c_int, c_int, c_int, c_int]),
         "sbk_default_config": (None, [ctypes.POINTER(Config)]),
         "sbk_default_fields": (None, [ctypes.POINTER(Fields)]),
         "sbk_create": (c_int, [ctypes.POINTER(Config), ctypes.POINTER(vp)]),
@@ -211,6 +210,35 @@ def lib():
         fn.restype = res
         fn.argtypes = args
     _lib = L
+    return L
+
+
+_nbody = None
+
+
+def nbody_lib():
+    """Loads libsimbuca_nbody.so (the reference's NBODY interface); raises if
+    it has not been built."""
+    global _nbody
+    if _nbody is not None:
+        return _nbody
+    lib()   # its Coulomb kernels live in libsimbuca_hip.so
+    if not os.path.exists(NBODY_LIB_PATH):
+        raise SimbucaError(f"{NBODY_LIB_PATH} is missing: build it with `make -C simbuca_amd/csrc`")
+    L = ctypes.CDLL(NBODY_LIB_PATH)
+    vp, c_int = ctypes.c_void_p, ctypes.c_int
+    sig = {
+        "initialize": (None, [c_int, ctypes.c_bool]),
+        "finalize": (ctypes.c_float, []),
+        "simulation_Async": (None, [vp, vp, c_int, c_int, c_int, c_int]),
+        "simulation_Async_begin": (None, [vp, vp, c_int, c_int, c_int, c_int]),
+        "simulation_Async_end": (None, [vp, vp, c_int, c_int, c_int, c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _nbody = L
     return L
 
 
@@ -606,7 +634,7 @@ def nbody_accel(pos):
     initialize, simulation_Async on FP32 float4 positions, wait, finalize.
     Returns the FP32 accelerations sum_j (x_j-x_i)(|r|^2+1e-7)^-1.5."""
     import ctypes as C
-    L = lib()
+    L = nbody_lib()
     p = np.zeros((len(pos), 4), dtype=np.float32)
     p[:, :3] = np.asarray(pos, dtype=np.float32).reshape(-1, 3)
     p[:, 3] = 1.0                                     # w = 1 "charge" (bin/nbody.cpp:121)

@@ -18,7 +18,7 @@
 #include <cstdlib>
 #include <vector>
 
-#include "../../include/simbuca_hip.h"
+#include "../../include/simbuca_nbody.h"
 #include "sbk_internal.h"
 
 namespace {

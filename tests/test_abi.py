@@ -15,8 +15,8 @@ from simbuca_amd import synth
 from conftest import ROOT
 
 
-def _declared_symbols():
-    src = open(os.path.join(ROOT, "include", "simbuca_hip.h")).read()
+def _declared_symbols(header="simbuca_hip.h"):
+    src = open(os.path.join(ROOT, "include", header)).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     names = re.findall(r"^\s*(?:void|int|float|const char)\s*\*?\s*(\w+)\s*\(", src, flags=re.M)
     return sorted(set(names))
@@ -30,6 +30,19 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(S.LIB_PATH)
     for name in _declared_symbols():
         assert hasattr(lib, name), name
+
+
+def test_nbody_interface_is_a_library_of_its_own():
+    # the reference's NBODY names (bin/nbody.cpp:12-18) are generic, so they
+    # live in libsimbuca_nbody.so and the engine library exports none of them
+    assert _declared_symbols("simbuca_nbody.h") == sorted(S.NBODY_EXPORTS)
+    nb = ctypes.CDLL(S.NBODY_LIB_PATH)
+    for name in S.NBODY_EXPORTS:
+        assert hasattr(nb, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", S.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout.split()
+    for name in S.NBODY_EXPORTS:
+        assert name not in out, name
 
 
 def _c_layout(tmp_path):
@@ -126,8 +139,8 @@ def test_cxx_caller_links_against_the_header_only():
      "cunbody1_force_begin: bad arguments"),
     ("L.cunbody1_force_end(None, None, None, ctypes.c_double(0.0), None, 4, 4)",
      "no matching cunbody1_force_begin"),
-    ("L.initialize(0, False)", "n_max must be > 0"),
-    ("L.simulation_Async(None, None, 8, 0, 0, 0)", "initialize() was not called"),
+    ("N.initialize(0, False)", "n_max must be > 0"),
+    ("N.simulation_Async(None, None, 8, 0, 0, 0)", "initialize() was not called"),
 ])
 def test_reference_abi_errors_end_the_process(call, msg):
     # the reference backends' void entry points report errors by printing and
@@ -135,7 +148,8 @@ def test_reference_abi_errors_end_the_process(call, msg):
     # caught before any device call
     import subprocess
     import sys
-    code = f"import ctypes; L = ctypes.CDLL({S.LIB_PATH!r}); {call}; print('returned')"
+    code = (f"import ctypes; L = ctypes.CDLL({S.LIB_PATH!r}); N = ctypes.CDLL({S.NBODY_LIB_PATH!r}); "
+            f"{call}; print('returned')")
     p = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert p.returncode == 1, (p.returncode, p.stdout, p.stderr)
     assert msg in p.stderr and "returned" not in p.stdout
