@@ -163,6 +163,21 @@ typedef struct sbk_fields {
     const double *swift;
     int    swift_n, swift_kind;
     double swift_t0, swift_dt, swift_r;
+    /* Non-ideal trap (the upstream c4, c6 terms; used with V0 != 0): the trap
+     * potential is (V0/2) sum_k C_k (rho/d0)^k P_k(cos theta) for k = 2, 4, 6
+     * with C_2 = 1 (the ideal term above), that is
+     *   + V0 C4 / (2 d0^4) (z^4 - 3 z^2 r^2 + 3 r^4/8)
+     *   + V0 C6 / (2 d0^6) (z^6 - 15/2 z^4 r^2 + 45/8 z^2 r^4 - 5/16 r^6),  r^2 = x^2 + y^2. */
+    double C4, C6;
+    /* Axisymmetric electric-field map (the upstream field-map files): E_r and
+     * E_z in V/m on a grid r_i = i emap_dr (i < emap_nr), z_j = emap_z0 +
+     * j emap_dz (j < emap_nz), stored emap[2 (i emap_nz + j)] = E_r,
+     * emap[2 (i emap_nz + j) + 1] = E_z; bilinearly interpolated, 0 outside
+     * the grid, times emap_scale; E = (E_r x/r, E_r y/r, E_z).  The engine
+     * copies the grid when the fields are set; off when emap_nr is 0. */
+    const double *emap;
+    int    emap_nr, emap_nz;
+    double emap_dr, emap_z0, emap_dz, emap_scale;
 } sbk_fields;
 
 typedef struct sbk_config {

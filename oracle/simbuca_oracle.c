@@ -78,6 +78,31 @@ static double swift_amp(const orc_fields *f, double t)
     return a + w * (b - a);
 }
 
+/* field map at (r, z): bilinear interpolation of the (E_r, E_z) grid, 0 outside */
+static void emap_field(const orc_fields *f, double r, double z, double *er, double *ez)
+{
+    *er = 0.0;
+    *ez = 0.0;
+    double u = r / f->emap_dr, v = (z - f->emap_z0) / f->emap_dz;
+    if (!(u >= 0.0) || u > (double)(f->emap_nr - 1) || !(v >= 0.0) || v > (double)(f->emap_nz - 1))
+        return;
+    int i = (int)u, j = (int)v;
+    if (i > f->emap_nr - 2) i = f->emap_nr - 2;
+    if (j > f->emap_nz - 2) j = f->emap_nz - 2;
+    double wu = u - (double)i, wv = v - (double)j;
+    long long nz = f->emap_nz;
+    const double *e00 = f->emap + 2 * ((long long)i * nz + j);
+    const double *e01 = e00 + 2, *e10 = e00 + 2 * nz, *e11 = e10 + 2;
+    double out[2];
+    for (int k = 0; k < 2; k++) {
+        double lo = e00[k] + wu * (e10[k] - e00[k]);
+        double hi = e01[k] + wu * (e11[k] - e01[k]);
+        out[k] = lo + wv * (hi - lo);
+    }
+    *er = out[0];
+    *ez = out[1];
+}
+
 /* bin/force.cpp:34-82 (trap/TOF) followed by the extension terms and then
  * bin/force.cpp:147-175 (Coulomb).  derivs is an acceleration. */
 void orc_force(int n, const double *pos_, const double *vel_, const double *mass,
@@ -115,6 +140,10 @@ void orc_force(int n, const double *pos_, const double *vel_, const double *mass
     double invK = (f->K0 != 0.0 && f->p_mbar != 0.0)
                       ? (f->p_mbar / 1013.25) * (273.15 / f->T_gas) / f->K0 : 0.0;
     double uswift = (f->swift_n > 0) ? swift_amp(f, t) : 0.0;
+    double c4k = (f->V0 != 0.0 && f->C4 != 0.0)
+                     ? f->V0 * f->C4 / (2.0 * f->d0 * f->d0 * f->d0 * f->d0) : 0.0;
+    double c6k = (f->V0 != 0.0 && f->C6 != 0.0)
+                     ? f->V0 * f->C6 / (2.0 * f->d0 * f->d0 * f->d0 * f->d0 * f->d0 * f->d0) : 0.0;
 
     for (int j_ = 0; j_ < n; j_++) {
         double x = pos[j_][0], y = pos[j_][1], z = pos[j_][2];
@@ -188,6 +217,35 @@ void orc_force(int n, const double *pos_, const double *vel_, const double *mass
                 derivs[j_][0] += qDmassa * cs * x;
                 derivs[j_][1] -= qDmassa * cs * y;
             }
+        }
+        if (c4k != 0.0 || c6k != 0.0) {
+            /* -grad of c4k r^4 P4 and c6k r^6 P6 (see the header) */
+            double r2 = x * x + y * y, z2 = z * z;
+            if (c4k != 0.0) {
+                double cr = c4k * (6.0 * z2 - 1.5 * r2);
+                double cz = c4k * (6.0 * r2 - 4.0 * z2);
+                derivs[j_][0] += qDmassa * cr * x;
+                derivs[j_][1] += qDmassa * cr * y;
+                derivs[j_][2] += qDmassa * cz * z;
+            }
+            if (c6k != 0.0) {
+                double z4 = z2 * z2, r4 = r2 * r2, zr = z2 * r2;
+                double cr = c6k * (15.0 * z4 - 22.5 * zr + 1.875 * r4);
+                double cz = c6k * (30.0 * zr - 6.0 * z4 - 11.25 * r4);
+                derivs[j_][0] += qDmassa * cr * x;
+                derivs[j_][1] += qDmassa * cr * y;
+                derivs[j_][2] += qDmassa * cz * z;
+            }
+        }
+        if (f->emap_nr > 0) {
+            double r = sqrt(x * x + y * y), er, ez;
+            emap_field(f, r, z, &er, &ez);
+            double g = qDmassa * f->emap_scale;
+            if (r > 0.0) {
+                derivs[j_][0] += g * (er * (x / r));
+                derivs[j_][1] += g * (er * (y / r));
+            }
+            derivs[j_][2] += g * ez;
         }
     }
 

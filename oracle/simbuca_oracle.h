@@ -15,9 +15,10 @@
  *   bin/ion_fly.cpp:83-101      move_particles loop incl. trap ramp
  *
  * Extensions that the fork does not contain (Penning B field, ideal
- * Penning quadrupole, dipole/quadrupole RF excitation, buffer-gas
- * friction, stage-time evaluation) are defined here from their closed
- * forms; their tests pin them against analytic solutions.
+ * Penning quadrupole and its C4/C6 anharmonic terms, field maps,
+ * dipole/quadrupole RF excitation, buffer-gas friction, stage-time
+ * evaluation) are defined here from their closed forms; their tests pin
+ * them against analytic solutions.
  */
 #ifndef SIMBUCA_ORACLE_H
 #define SIMBUCA_ORACLE_H
@@ -68,6 +69,14 @@ typedef struct orc_fields {
     const double *swift;
     int    swift_n, swift_kind;
     double swift_t0, swift_dt, swift_r;
+    /* non-ideal trap: the potential adds V0 C4/(2 d0^4) r^4 P4 and V0 C6/(2 d0^6) r^6 P6
+     * (Legendre solid harmonics; used with V0 != 0) */
+    double C4, C6;
+    /* axisymmetric field map: (E_r, E_z) on r_i = i dr, z_j = z0 + j dz, emap[2 (i nz + j) + {0,1}],
+     * bilinear, 0 outside the grid, times emap_scale */
+    const double *emap;
+    int    emap_nr, emap_nz;
+    double emap_dr, emap_z0, emap_dz, emap_scale;
 } orc_fields;
 
 enum { ORC_TIME_FROZEN = 0, ORC_TIME_STAGE = 1 };

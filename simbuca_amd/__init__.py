@@ -73,6 +73,10 @@ class Fields(ctypes.Structure):
         ("swift", ctypes.POINTER(ctypes.c_double)), ("swift_n", ctypes.c_int),
         ("swift_kind", ctypes.c_int), ("swift_t0", ctypes.c_double), ("swift_dt", ctypes.c_double),
         ("swift_r", ctypes.c_double),
+        ("C4", ctypes.c_double), ("C6", ctypes.c_double),
+        ("emap", ctypes.POINTER(ctypes.c_double)), ("emap_nr", ctypes.c_int),
+        ("emap_nz", ctypes.c_int), ("emap_dr", ctypes.c_double), ("emap_z0", ctypes.c_double),
+        ("emap_dz", ctypes.c_double), ("emap_scale", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -278,6 +282,22 @@ def set_swift(f, samples, t0, dt, r, kind=1):
     f.swift_n = int(a.size)
     f.swift_kind = int(kind)
     f.swift_t0, f.swift_dt, f.swift_r = float(t0), float(dt), float(r)
+    return f
+
+
+def set_emap(f, er, ez, dr, z0, dz, scale=1.0):
+    """Attaches an axisymmetric field map to fields f: er, ez [nr, nz] in V/m
+    at r = i dr, z = z0 + j dz; bilinear, 0 outside, times scale.  The
+    fields object keeps the grid alive."""
+    er = np.asarray(er, dtype=np.float64)
+    ez = np.asarray(ez, dtype=np.float64)
+    if er.ndim != 2 or er.shape != ez.shape:
+        raise ValueError("er and ez must be [nr, nz] grids of one shape")
+    g = np.ascontiguousarray(np.stack([er, ez], axis=-1)).ravel()
+    f._emap_keep = g
+    f.emap = g.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    f.emap_nr, f.emap_nz = int(er.shape[0]), int(er.shape[1])
+    f.emap_dr, f.emap_z0, f.emap_dz, f.emap_scale = float(dr), float(z0), float(dz), float(scale)
     return f
 
 

@@ -72,6 +72,7 @@ struct HostCloud {
     std::vector<int> charge;
     std::vector<int> perm;                // slot -> caller's index (empty: identity)
     std::vector<double> swift;            // SWIFT samples, copied from the fields
+    std::vector<double> emap;             // field map (E_r, E_z per node), copied from the fields
     int xcross_on = 0;                    // record crossings of the plane x = xcross
     double xcross = 0.0;
 };
@@ -201,6 +202,16 @@ sbk::CloudParams make_params(const sbk_fields &f, int off, int n)
     p.swift_t0 = f.swift_t0;
     p.swift_dt = f.swift_dt;
     p.swift_r = f.swift_r;
+    p.c4k = (f.V0 != 0.0 && f.C4 != 0.0) ? f.V0 * f.C4 / (2.0 * f.d0 * f.d0 * f.d0 * f.d0) : 0.0;
+    p.c6k = (f.V0 != 0.0 && f.C6 != 0.0)
+                ? f.V0 * f.C6 / (2.0 * f.d0 * f.d0 * f.d0 * f.d0 * f.d0 * f.d0) : 0.0;
+    p.emap = nullptr;    // set by cloud_params
+    p.emap_nr = f.emap_nr;
+    p.emap_nz = f.emap_nz;
+    p.emap_dr = f.emap_dr;
+    p.emap_z0 = f.emap_z0;
+    p.emap_dz = f.emap_dz;
+    p.emap_scale = f.emap_scale;
     p.off = off;
     p.n = n;
     p.slices = 1;
@@ -434,6 +445,7 @@ sbk::CloudParams cloud_params(const sbk_engine *e, size_t ci)
     const bool timed = (c.f.trap && c.f.Vrf != 0.0) || c.f.Ed != 0.0 || c.f.Uq != 0.0 ||
                        c.f.Uo != 0.0 || c.f.Uw != 0.0 || c.f.Eax != 0.0 || c.f.swift_n > 0;
     if (c.f.swift_n > 0 && e->d_swift) p.swift = e->d_swift + e->swift_off[ci];
+    if (c.f.emap_nr > 0 && e->d_swift) p.emap = e->d_swift + e->swift_off[ci] + c.swift.size();
     p.fsal = !c.f.ramp && (e->cfg.time_mode == SBK_TIME_STAGE || !timed);
     p.xcross_on = c.xcross_on;
     p.xcross = c.xcross;
@@ -507,8 +519,9 @@ int drain_crossings(sbk_engine *e)
     return 0;
 }
 
-// Concatenates every cloud's SWIFT samples into one device buffer (grown as
-// needed; the stream must be idle, since the old buffer may be freed).
+// Concatenates every cloud's SWIFT samples, each followed by its field map,
+// into one device buffer (grown as needed; the stream must be idle, since
+// the old buffer may be freed).
 int upload_swift(sbk_engine *e)
 {
     std::vector<double> all;
@@ -516,6 +529,7 @@ int upload_swift(sbk_engine *e)
     for (size_t ci = 0; ci < e->clouds.size(); ++ci) {
         e->swift_off[ci] = all.size();
         all.insert(all.end(), e->clouds[ci].swift.begin(), e->clouds[ci].swift.end());
+        all.insert(all.end(), e->clouds[ci].emap.begin(), e->clouds[ci].emap.end());
     }
     if (all.empty()) return 0;
     if (all.size() > e->swift_cap) {
@@ -1013,6 +1027,14 @@ static int check_fields(const sbk_fields *f)
                                              (f->swift_kind != 1 && f->swift_kind != 2))))
         return fail(SBK_EINVAL, "SWIFT needs >= 2 samples, swift_dt > 0, swift_r > 0 and "
                                 "swift_kind 1 or 2");
+    if (f->emap_nr < 0 || (f->emap_nr > 0 && (!f->emap || f->emap_nr < 2 || f->emap_nz < 2 ||
+                                              !(f->emap_dr > 0.0) || !(f->emap_dz > 0.0) ||
+                                              !std::isfinite(f->emap_z0) ||
+                                              (long long)f->emap_nr * f->emap_nz > (1LL << 26))))
+        return fail(SBK_EINVAL, "a field map needs 2 <= emap_nr, 2 <= emap_nz, "
+                                "emap_nr * emap_nz <= 2^26, emap_dr > 0, emap_dz > 0");
+    if ((f->C4 != 0.0 || f->C6 != 0.0) && f->V0 == 0.0)
+        return fail(SBK_EINVAL, "C4 and C6 scale the Penning potential: set V0 and d0");
     return 0;
 }
 
@@ -1040,6 +1062,8 @@ int sbk_add_cloud(sbk_engine *e, int n, const double *pos, const double *vel, co
             return fail(SBK_EINVAL, "ion %zu has a non-finite position or velocity", i / 3);
     if (f->swift_n > 0) c.swift.assign(f->swift, f->swift + f->swift_n);
     c.f.swift = nullptr;   // the samples live in c.swift from here on
+    if (f->emap_nr > 0) c.emap.assign(f->emap, f->emap + 2 * (size_t)f->emap_nr * f->emap_nz);
+    c.f.emap = nullptr;    // and the map in c.emap
     e->clouds.push_back(std::move(c));
     e->ntot += n;
     if (cloud_id) *cloud_id = (int)e->clouds.size() - 1;
@@ -1057,6 +1081,8 @@ int sbk_set_fields(sbk_engine *e, int cloud, const sbk_fields *f)
     c.f = *f;
     c.swift.assign(f->swift, f->swift + (f->swift_n > 0 ? f->swift_n : 0));
     c.f.swift = nullptr;
+    c.emap.assign(f->emap, f->emap + (f->emap_nr > 0 ? 2 * (size_t)f->emap_nr * f->emap_nz : 0));
+    c.f.emap = nullptr;
     if (!e->finalized) return 0;
     HIP_TRY(hipSetDevice(e->device));
     HIP_TRY(hipStreamSynchronize(e->stream));   // the sample buffer may be reallocated

@@ -49,6 +49,10 @@ struct CloudParams {
     const double *swift;      // SWIFT samples on the device (swift_n > 0)
     int swift_n, swift_kind;
     double swift_t0, swift_dt, swift_r;
+    double c4k, c6k;          // anharmonic trap: V0 C4 / (2 d0^4), V0 C6 / (2 d0^6)
+    const double *emap;       // field map on the device (emap_nr > 0): (E_r, E_z) per node
+    int emap_nr, emap_nz;
+    double emap_dr, emap_z0, emap_dz, emap_scale;
     // layout
     int off, n, slices, pad;
     int blk0, nblk;           // this cloud's range in the ion-block table

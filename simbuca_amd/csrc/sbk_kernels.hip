@@ -710,6 +710,35 @@ __device__ __forceinline__ double swift_amp(const CloudParams &p, double t)
     return a + w * (b - a);
 }
 
+// Axisymmetric field map at (r, z): bilinear interpolation of the (E_r, E_z)
+// grid, 0 outside it.  The cell index is clamped to the last cell, so the
+// four nodes read are always inside the map.
+__device__ __forceinline__ void emap_field(const CloudParams &p, double r, double z, double &er,
+                                           double &ez)
+{
+    er = 0.0;
+    ez = 0.0;
+    const double u = r / p.emap_dr, v = (z - p.emap_z0) / p.emap_dz;
+    if (!(u >= 0.0) || u > (double)(p.emap_nr - 1) || !(v >= 0.0) || v > (double)(p.emap_nz - 1))
+        return;
+    int i = (int)u, j = (int)v;
+    if (i > p.emap_nr - 2) i = p.emap_nr - 2;
+    if (j > p.emap_nz - 2) j = p.emap_nz - 2;
+    const double wu = u - (double)i, wv = v - (double)j;
+    const long long nz = p.emap_nz;
+    const double *e00 = p.emap + 2 * ((long long)i * nz + j);
+    const double *e01 = e00 + 2, *e10 = e00 + 2 * nz, *e11 = e10 + 2;
+    double out[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double lo = e00[k] + wu * (e10[k] - e00[k]);
+        const double hi = e01[k] + wu * (e11[k] - e01[k]);
+        out[k] = lo + wv * (hi - lo);
+    }
+    er = out[0];
+    ez = out[1];
+}
+
 // force() at one ion (bin/force.cpp:58-105 order), then the extension terms,
 // then the Coulomb subtraction.  t_life is IonCloud::lifetime (drives the
 // trap ramp, bin/ion_fly.cpp:235-239); ts is the field evaluation time.
@@ -803,6 +832,37 @@ __device__ __forceinline__ void ion_accel(const CloudParams &p, double t_life, d
             ax += qm * cs * x;
             ay -= qm * cs * y;
         }
+    }
+    if (p.c4k != 0.0 || p.c6k != 0.0) {
+        // anharmonic trap: -grad of c4k (z^4 - 3 z^2 r^2 + 3 r^4/8) and
+        // c6k (z^6 - 15/2 z^4 r^2 + 45/8 z^2 r^4 - 5/16 r^6)
+        const double r2 = x * x + y * y, z2 = z * z;
+        if (p.c4k != 0.0) {
+            const double cr = p.c4k * (6.0 * z2 - 1.5 * r2);
+            const double cz = p.c4k * (6.0 * r2 - 4.0 * z2);
+            ax += qm * cr * x;
+            ay += qm * cr * y;
+            az += qm * cz * z;
+        }
+        if (p.c6k != 0.0) {
+            const double z4 = z2 * z2, r4 = r2 * r2, zr = z2 * r2;
+            const double cr = p.c6k * (15.0 * z4 - 22.5 * zr + 1.875 * r4);
+            const double cz = p.c6k * (30.0 * zr - 6.0 * z4 - 11.25 * r4);
+            ax += qm * cr * x;
+            ay += qm * cr * y;
+            az += qm * cz * z;
+        }
+    }
+    if (p.emap_nr > 0) {
+        const double r = sqrt(x * x + y * y);
+        double er, ez;
+        emap_field(p, r, z, er, ez);
+        const double g = qm * p.emap_scale;
+        if (r > 0.0) {
+            ax += g * (er * (x / r));
+            ay += g * (er * (y / r));
+        }
+        az += g * ez;
     }
     ax -= Fx;
     ay -= Fy;

@@ -43,19 +43,6 @@ Plan plan_coulomb(const std::vector<PlanCloud> &clouds, int blk, int quantum, in
         // slice boundaries (relative to the cloud's first ion)
         std::vector<int> cut;
         for (int s = 0; s <= S; ++s) cut.push_back(std::min(c.n, s * chunk));
-#ifdef SBK_TAPER_F
-        if (!exact && sreq <= 0 && S >= 8 && chunk % (2 * quantum) == 0) {
-            // the first part of the sources in double-length slices, the last
-            // SBK_TAPER_F of them in half-length ones
-            const int tail = ((int)(SBK_TAPER_F * c.n) / chunk) * chunk;
-            cut.clear();
-            int j = 0;
-            cut.push_back(0);
-            while (j < c.n - tail) { j = std::min(c.n - tail, j + 2 * chunk); cut.push_back(j); }
-            while (j < c.n) { j = std::min(c.n, j + chunk / 2); cut.push_back(j); }
-            S = (int)cut.size() - 1;
-        }
-#endif
         L.slices[ci] = S;
         L.max_slices = std::max(L.max_slices, S);
         for (int s = 0; s < S; ++s) {

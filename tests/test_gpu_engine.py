@@ -105,6 +105,81 @@ def test_field_extras_bit_identical(rng):
     assert np.array_equal(out, O.force(pos, vel, m, ch, of, 1.1e-6))
 
 
+def _field_grid(nr=60, nz=91, dr=1e-4, z0=-4.5e-3, dz=1e-4, a=3e4, b=-2e4):
+    """A nonlinear axisymmetric test field (E_r, E_z) [nr, nz] and its grid."""
+    r = np.arange(nr) * dr
+    z = z0 + np.arange(nz) * dz
+    R, Z = np.meshgrid(r, z, indexing="ij")
+    er = a * R * (1.0 + 300.0 * Z) + 1e6 * R ** 2
+    ez = b * Z + 5e5 * R * R - 4e6 * Z ** 3
+    return er, ez, dr, z0, dz
+
+
+def test_anharmonic_trap_and_field_map_bit_identical(rng):
+    # C4/C6 terms and a field map (inside and outside the grid) have no
+    # cosine: the force is bit-identical to the oracle's, and a short adaptive
+    # run follows the oracle's trajectory
+    amu, m, ch, pos, vel = _rand_cloud(rng, 600, sigma=2e-3)
+    kw = dict(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0, C4=0.03, C6=-0.02)
+    sf, of = _both_fields(**kw)
+    er, ez, dr, z0, dz = _field_grid()
+    S.set_emap(sf, er, ez, dr, z0, dz, scale=0.7)
+    O.set_emap(of, er, ez, dr, z0, dz, scale=0.7)
+    with S.Engine(precision="fp64_exact", rtol=1e-9, atol=1e-9) as e:
+        e.add_cloud(pos, vel, m, ch, sf, lifetime=2e-7)
+        out = e.eval_force(0)
+        for _ in range(25):
+            e.step()
+        gp, gv = e.state(0)
+        st = e.status(0)
+    assert np.array_equal(out, O.force(pos, vel, m, ch, of, 2e-7))
+    c = O.Cloud(pos, vel, m, ch, of, lifetime=2e-7, h0=1e-9, rtol=1e-9, atol=1e-9)
+    for _ in range(25):
+        c.step()
+    op, ov, sc = c.state()
+    assert st["attempts"] == sc["attempts"]
+    assert np.allclose(gp, op, rtol=1e-10, atol=1e-10 * np.abs(op).max())
+    assert np.allclose(gv, ov, rtol=1e-10, atol=1e-10 * np.abs(ov).max())
+
+
+def test_field_maps_and_swift_share_the_sample_buffer(rng):
+    # two clouds, one with a SWIFT waveform and a map, one with a larger map;
+    # then the first cloud's map is replaced: every force equals the oracle's
+    sizes = (200, 300)
+    clouds, fields = [], []
+    g1 = _field_grid(nr=20, nz=31, dr=3e-4, z0=-4.5e-3, dz=3e-4)
+    g2 = _field_grid(nr=70, nz=101, dr=1e-4, z0=-5e-3, dz=1e-4, a=-2e4, b=5e4)
+    for k, n in enumerate(sizes):
+        amu, m, ch, pos, vel = _rand_cloud(rng, n, sigma=1e-3)
+        sf, of = _both_fields(trap=0, coulomb=1, V0=10.0, d0=11e-3, C4=0.01 * (k + 1))
+        g = g1 if k == 0 else g2
+        S.set_emap(sf, *g); O.set_emap(of, *g)
+        if k == 0:
+            w = np.sin(np.linspace(0, 9, 40))
+            S.set_swift(sf, w, 0.0, 1e-8, 5e-3, kind=2); O.set_swift(of, w, 0.0, 1e-8, 5e-3, kind=2)
+        clouds.append((pos, vel, m, ch)); fields.append((sf, of))
+    with S.Engine(precision="fp64_exact") as e:
+        for (pos, vel, m, ch), (sf, _) in zip(clouds, fields):
+            e.add_cloud(pos, vel, m, ch, sf, lifetime=1.3e-7)
+        for k in range(2):
+            pos, vel, m, ch = clouds[k]
+            ref = O.force(pos, vel, m, ch, fields[k][1], 1.3e-7)
+            assert np.allclose(e.eval_force(k), ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max()), k
+        sf, of = _both_fields(trap=0, coulomb=1, V0=10.0, d0=11e-3, C6=0.05)
+        S.set_emap(sf, *g2); O.set_emap(of, *g2)
+        e.set_fields(0, sf)
+        for k, f_of in ((0, of), (1, fields[1][1])):
+            pos, vel, m, ch = clouds[k]
+            ref = O.force(pos, vel, m, ch, f_of, 1.3e-7)
+            assert np.array_equal(e.eval_force(k), ref), k
+    # a malformed map is refused
+    bad = S.set_emap(S.default_fields(trap=0), *g1)
+    bad.emap_dz = 0.0
+    with S.Engine(precision="mixed") as e:
+        with pytest.raises(S.SimbucaError, match="field map"):
+            e.add_cloud(*clouds[0], bad)
+
+
 @pytest.mark.parametrize("order", [1, 2])
 def test_excitation_menu_matches_oracle(rng, order):
     # octupole, rotating wall, axial drive and swept dipole/quadrupole drives

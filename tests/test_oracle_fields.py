@@ -326,3 +326,150 @@ def test_swift_driven_free_ion():
     pos, vel, sc = c.state()
     tt = sc["lifetime"]
     assert pos[0, 0] == pytest.approx(A * (1 - math.cos(w * tt)) / w ** 2, rel=1e-5)
+
+
+# ---------------------------------------------------------------------------
+# non-ideal trap (C4, C6) and axisymmetric field maps
+# ---------------------------------------------------------------------------
+V0, D0 = 10.0, 11e-3
+
+
+def _legendre_phi(k, Ck):
+    """(V0/2) C_k (rho/d0)^k P_k(cos theta) from scipy's Legendre polynomials:
+    an independent route to the potential whose gradient the oracle uses."""
+    from scipy.special import eval_legendre
+
+    def phi(p):
+        rho = float(np.linalg.norm(p))
+        return 0.5 * V0 * Ck * (rho / D0) ** k * eval_legendre(k, p[2] / rho)
+    return phi
+
+
+@pytest.mark.parametrize("k,Ck", [(4, 0.03), (6, -0.02)])
+def test_anharmonic_terms_are_minus_grad_of_the_legendre_potential(rng, k, Ck):
+    kw = {"C4": Ck} if k == 4 else {"C6": Ck}
+    pts = rng.uniform(-4e-3, 4e-3, (12, 3))
+    E_all = _field_at(pts, V0=V0, d0=D0, **kw)
+    E_ideal = _field_at(pts, V0=V0, d0=D0)
+    phi = _legendre_phi(k, Ck)
+    for p, e in zip(pts, E_all - E_ideal):
+        want = -_grad(phi, p, h=3e-7)
+        # central differences err by about (h/rho)^2; a slipped coefficient errs by O(1)
+        assert np.allclose(e, want, rtol=1e-5, atol=1e-6 * np.abs(want).max()), (k, p, e, want)
+
+
+@pytest.mark.parametrize("kw", [{"C4": 0.05}, {"C6": 0.05}, {"C4": -0.01, "C6": 0.02}])
+def test_anharmonic_field_is_divergence_free(rng, kw):
+    # a potential field in vacuum: div E = 0 (the Laplace condition that fixes
+    # the radial coefficients once the axial ones are given)
+    h = 1e-6
+    for p in rng.uniform(-4e-3, 4e-3, (8, 3)):
+        div, scale = 0.0, 0.0
+        for k in range(3):
+            d = np.zeros(3); d[k] = h
+            dk = (_field_at([p + d], V0=V0, d0=D0, **kw)[0, k]
+                  - _field_at([p - d], V0=V0, d0=D0, **kw)[0, k]) / (2 * h)
+            div += dk
+            scale = max(scale, abs(dk))
+        assert abs(div) <= 1e-6 * scale, (p, div, scale)
+
+
+def _axial_frequency(C4=0.0, C6=0.0, A=2e-3, periods=10, per_period=400):
+    """Angular frequency of an ion released at rest at z = A on the axis,
+    from the zero crossings of z(t) integrated by the oracle's DP5."""
+    m = O.mass_kg([133.0])[0]
+    w0 = math.sqrt(E * V0 / (m * D0 * D0))
+    h = 2 * math.pi / w0 / per_period
+    p, v, mm, ch = _single([0.0, 0.0, A], [0.0, 0.0, 0.0])
+    c = O.Cloud(p, v, mm, ch, O.fields(trap=0, V0=V0, d0=D0, C4=C4, C6=C6), h0=h, adaptive=False)
+    ts, zs = [0.0], [A]
+    for _ in range(periods * per_period):
+        c.step()
+        pos, _, sc = c.state()
+        ts.append(sc["lifetime"]); zs.append(pos[0, 2])
+    ts, zs = np.array(ts), np.array(zs)
+    k = np.nonzero(np.sign(zs[:-1]) != np.sign(zs[1:]))[0]
+    tc = ts[k] - zs[k] * (ts[k + 1] - ts[k]) / (zs[k + 1] - zs[k])
+    half = np.diff(tc).mean()
+    return math.pi / half, w0
+
+
+@pytest.mark.parametrize("C4,C6,A,want", [
+    (0.02, 0.0, 2e-3, lambda A: 0.75 * 0.02 * (A / D0) ** 2),
+    (0.0, 0.1, 3e-3, lambda A: 15.0 / 16.0 * 0.1 * (A / D0) ** 4),
+    (-0.03, 0.0, 2.5e-3, lambda A: 0.75 * -0.03 * (A / D0) ** 2),
+])
+def test_axial_frequency_shifts_with_amplitude(C4, C6, A, want):
+    # on the axis z'' = -w0^2 (z + 2 C4 z^3/d0^2 + 3 C6 z^5/d0^4); harmonic
+    # balance gives dw/w0 = (3/4) C4 (A/d0)^2 and (15/16) C6 (A/d0)^4 to first order
+    w, w0 = _axial_frequency(C4, C6, A)
+    w_ideal, _ = _axial_frequency(0.0, 0.0, A)
+    assert abs(w_ideal / w0 - 1.0) < 1e-9                # the ideal trap: exactly w0
+    shift = w / w0 - 1.0
+    assert shift == pytest.approx(want(A), rel=0.02), (shift, want(A))
+
+
+def test_anharmonic_radial_force_in_the_midplane():
+    # z = 0: phi4 = c4k (3/8) r^4 and phi6 = -c6k (5/16) r^6, so the radial field is
+    # -(3/2) c4k r^3 and +(15/8) c6k r^5
+    r = 3e-3
+    c4k = V0 * 0.04 / (2 * D0 ** 4)
+    c6k = V0 * 0.07 / (2 * D0 ** 6)
+    e4 = _field_at([[r, 0.0, 0.0]], V0=V0, d0=D0, C4=0.04)[0] - _field_at([[r, 0.0, 0.0]], V0=V0, d0=D0)[0]
+    e6 = _field_at([[0.0, r, 0.0]], V0=V0, d0=D0, C6=0.07)[0] - _field_at([[0.0, r, 0.0]], V0=V0, d0=D0)[0]
+    assert e4[0] == pytest.approx(-1.5 * c4k * r ** 3, rel=1e-9) and abs(e4[2]) == 0.0
+    assert e6[1] == pytest.approx(1.875 * c6k * r ** 5, rel=1e-9) and abs(e6[2]) == 0.0
+
+
+def _grid(nr, nz, dr, z0, dz, field):
+    r = np.arange(nr) * dr
+    z = z0 + np.arange(nz) * dz
+    R, Z = np.meshgrid(r, z, indexing="ij")
+    er, ez = field(R, Z)
+    return er, ez
+
+
+def test_field_map_of_a_linear_field_is_exact(rng):
+    # the ideal quadrupole E = cpen (r, -2 z) is linear in r and in z, which
+    # bilinear interpolation reproduces: the map equals the V0 term
+    cpen = V0 / (2 * D0 * D0)
+    er, ez = _grid(41, 61, 2.5e-4, -7.5e-3, 2.5e-4, lambda R, Z: (cpen * R, -2.0 * cpen * Z))
+    f = O.set_emap(O.fields(trap=0), er, ez, 2.5e-4, -7.5e-3, 2.5e-4)
+    pts = np.column_stack([rng.uniform(-6e-3, 6e-3, (50, 2)), rng.uniform(-7e-3, 7e-3, 50)])
+    pts = pts[np.hypot(pts[:, 0], pts[:, 1]) < 9.5e-3]
+    m = O.mass_kg(np.full(len(pts), 133.0)); ch = np.ones(len(pts), np.int32)
+    a_map = O.force(pts, np.zeros_like(pts), m, ch, f, 0.0)
+    a_v0 = O.force(pts, np.zeros_like(pts), m, ch, O.fields(trap=0, V0=V0, d0=D0), 0.0)
+    assert np.allclose(a_map, a_v0, rtol=1e-11, atol=1e-11 * np.abs(a_v0).max())
+    # scale multiplies; outside the grid the map adds nothing; on the axis no radial part
+    f2 = O.set_emap(O.fields(trap=0), er, ez, 2.5e-4, -7.5e-3, 2.5e-4, scale=-2.0)
+    assert np.allclose(O.force(pts, np.zeros_like(pts), m, ch, f2, 0.0), -2.0 * a_map, rtol=1e-14)
+    out = np.array([[0.0, 0.0, 8e-3], [1.2e-2, 0.0, 0.0], [0.0, -1.1e-2, 1e-3]])
+    m3 = O.mass_kg(np.full(3, 133.0)); c3 = np.ones(3, np.int32)
+    assert np.array_equal(O.force(out, np.zeros((3, 3)), m3, c3, f, 0.0), np.zeros((3, 3)))
+    axis = np.array([[0.0, 0.0, 1e-3]])
+    a_axis = O.force(axis, np.zeros((1, 3)), O.mass_kg([133.0]), np.ones(1, np.int32), f, 0.0)
+    assert a_axis[0, 0] == 0.0 and a_axis[0, 1] == 0.0 and np.isfinite(a_axis).all()
+
+
+def test_field_map_converges_at_second_order():
+    # the C4 field (nonlinear) sampled on grids of spacing h and h/2: the
+    # interpolation error falls about fourfold
+    c4k = V0 * 0.05 / (2 * D0 ** 4)
+
+    def c4_field(R, Z):   # E = -grad phi4 in (r, z)
+        return c4k * R * (6.0 * Z * Z - 1.5 * R * R), c4k * Z * (6.0 * R * R - 4.0 * Z * Z)
+
+    g = np.random.Generator(np.random.PCG64(17))
+    pts = np.column_stack([g.uniform(-3e-3, 3e-3, (3000, 2)), g.uniform(-4e-3, 4e-3, 3000)])
+    m = O.mass_kg(np.full(len(pts), 133.0)); ch = np.ones(len(pts), np.int32)
+    exact = (O.force(pts, np.zeros_like(pts), m, ch, O.fields(trap=0, V0=V0, d0=D0, C4=0.05), 0.0)
+             - O.force(pts, np.zeros_like(pts), m, ch, O.fields(trap=0, V0=V0, d0=D0), 0.0))
+    errs = []
+    for h in (2e-4, 1e-4, 5e-5):
+        nr, nz = int(round(5e-3 / h)) + 1, int(round(1e-2 / h)) + 1
+        er, ez = _grid(nr, nz, h, -5e-3, h, c4_field)
+        f = O.set_emap(O.fields(trap=0), er, ez, h, -5e-3, h)
+        a = O.force(pts, np.zeros_like(pts), m, ch, f, 0.0)
+        errs.append(np.sqrt(((a - exact) ** 2).mean() / (exact ** 2).mean()))   # rms
+    assert errs[0] / errs[1] > 3.6 and errs[1] / errs[2] > 3.6, errs
