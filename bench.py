@@ -88,9 +88,9 @@ def best_table_shape(table, device, w, precision):
     return best[1] if best else None
 
 
-def make_engine(S, w, precision, timing, use_graph=True, tune=None):
+def make_engine(S, w, precision, timing, use_graph=True, tune=None, persistent=0):
     e = S.Engine(precision=precision, time_mode=w.time_mode, rtol=w.rtol, atol=w.atol, h0=w.h0,
-                 use_graph=use_graph, timing=timing)
+                 use_graph=use_graph, timing=timing, persistent=persistent)
     for c in w.clouds:
         e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
     if tune is not None:
@@ -122,10 +122,10 @@ def spin_up(e, precision, ms=100.0):
     e.time_coulomb(precision, max(1, min(2000, int(ms / max(first, 1e-3)))))
 
 
-def run_engine(S, w, precision, steps, warmup, timing, use_graph=True, tune=None):
+def run_engine(S, w, precision, steps, warmup, timing, use_graph=True, tune=None, persistent=0):
     """`warmup` then `steps` timed attempts of every cloud (inputs resident),
     after the clocks are spun up."""
-    e = make_engine(S, w, precision, timing, use_graph, tune)
+    e = make_engine(S, w, precision, timing, use_graph, tune, persistent)
     spin_up(e, precision)
     e.attempts(warmup)                  # activates with no end time, warms up
     e.synchronize()
@@ -372,6 +372,28 @@ def kernels_only(S, w, shape=None):
     log("copied 3 x 1 GiB")
 
 
+def replay(S, w, precision, steps, warmup, tune):
+    """Attempt throughput without events: the per-attempt graph replayed
+    `steps` times against one persistent launch of `steps` attempts (grid
+    barriers between the phases), where the launch shape has that kernel."""
+    out = {}
+    for name, pers in (("graph", 0), ("persistent", 1)):
+        try:
+            e, dt, st, _ = run_engine(S, w, precision, steps, warmup, timing=False, tune=tune,
+                                      persistent=pers)
+        except S.SimbucaError as ex:
+            out[name] = {"error": str(ex)}
+            continue
+        out[name] = {"attempts": steps, "seconds": dt, "us_per_attempt": 1e6 * dt / steps,
+                     "steps": st["steps"], "ion_steps_per_s": st["ion_steps"] / dt,
+                     "pair_interactions_per_s": st["pair_interactions"] / dt,
+                     "persistent_launches": st["persistent_launches"]}
+        e.close()
+    if "seconds" in out.get("graph", {}) and "seconds" in out.get("persistent", {}):
+        out["speedup"] = out["graph"]["seconds"] / out["persistent"]["seconds"]
+    return out
+
+
 def other_workload(S, w, args, precision):
     """Secondary workloads: attempt throughput of the mixed and FP64 engines
     (and optionally the whole simulated interval); one JSON line."""
@@ -396,6 +418,10 @@ def other_workload(S, w, args, precision):
                               "frac": tf / peak}
         e.close()
         log(f"{p}: {res[p]}")
+        if res["pairs_per_attempt"] < 7e8:
+            # launch-bound sizes: the whole attempt, not the Coulomb kernel, sets the rate
+            res[p]["replay"] = replay(S, w, p, max(args.steps, 400), args.warmup, tune)
+            log(f"{p} replay: {res[p]['replay']}")
         if args.to_end:
             dte, ste = run_to_end(S, w, p, tune)
             res[p]["to_end"] = {"t_sim": w.t_sim, "seconds": dte, "steps": ste["steps"],

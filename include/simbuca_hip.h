@@ -199,6 +199,17 @@ typedef struct sbk_config {
                            value, default) or e.g. 0.04..0.08, the reference's
                            documented switch (bin/ode.cpp:427-432); alpha =
                            0.2 - 0.75 beta.  Must be in [0, 0.2). */
+    int    persistent;  /* 1: each batch of attempts is one cooperative launch, with grid barriers
+                           between the phases, instead of the graph replay (16 launches per
+                           attempt); same results bit for bit.  0 (the default): the graph or
+                           separate launches.  On MI355X a grid barrier (2 us at 16 work-groups,
+                           13 us at 512) costs more than a graph's kernel boundary (1.7 us), so
+                           the graph is faster at every size measured.  Needs 4 source groups
+                           and 1 or 2 targets per thread (SBK_EINVAL otherwise) and a cooperative
+                           queue, which the device may refuse while another process holds it
+                           (SBK_EHIP); timing = 1 uses separate launches. */
+    int    persistent_spin; /* polls of a grid-barrier wait before the launch gives up and
+                           reports SBK_EHIP (0: the default, 4194304, seconds of waiting) */
 } sbk_config;
 
 typedef struct sbk_cloud_status {
@@ -220,6 +231,7 @@ typedef struct sbk_stats {
     long long steps;             /* accepted steps, summed over clouds */
     double    ion_steps;         /* sum over clouds of steps*n */
     long long reorders;          /* re-sorts of the device ion order (spatial_order) */
+    long long persistent_launches; /* batches of attempts run as one persistent launch */
 } sbk_stats;
 
 typedef struct sbk_engine sbk_engine;

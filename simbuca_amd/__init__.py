@@ -89,7 +89,7 @@ class Config(ctypes.Structure):
         ("use_graph", ctypes.c_int), ("timing", ctypes.c_int),
         ("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("h0", ctypes.c_double),
         ("spatial_order", ctypes.c_int), ("reorder_interval", ctypes.c_int),
-        ("beta", ctypes.c_double),
+        ("beta", ctypes.c_double), ("persistent", ctypes.c_int), ("persistent_spin", ctypes.c_int),
     ]
 
 
@@ -113,7 +113,7 @@ class Stats(ctypes.Structure):
         ("coulomb_ms", ctypes.c_double), ("coulomb_launches", ctypes.c_longlong),
         ("pair_interactions", ctypes.c_double), ("attempts", ctypes.c_longlong),
         ("steps", ctypes.c_longlong), ("ion_steps", ctypes.c_double),
-        ("reorders", ctypes.c_longlong),
+        ("reorders", ctypes.c_longlong), ("persistent_launches", ctypes.c_longlong),
     ]
 
     def as_dict(self):
@@ -475,7 +475,7 @@ class Engine:
 
     def __init__(self, precision="mixed", time_mode=TIME_FROZEN, adaptive=True, use_graph=True,
                  timing=False, rtol=1e-7, atol=1e-7, h0=1e-9, spatial_order=None,
-                 reorder_interval=None, beta=0.0):
+                 reorder_interval=None, beta=0.0, persistent=0, persistent_spin=0):
         L = lib()
         cfg = Config()
         L.sbk_default_config(ctypes.byref(cfg))
@@ -490,6 +490,8 @@ class Engine:
         if reorder_interval is not None:
             cfg.reorder_interval = int(reorder_interval)
         cfg.beta = float(beta)
+        cfg.persistent = int(persistent)
+        cfg.persistent_spin = int(persistent_spin)
         h = ctypes.c_void_p()
         _check(L.sbk_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

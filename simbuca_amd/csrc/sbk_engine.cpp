@@ -140,6 +140,13 @@ struct sbk_engine {
     int kernel_req = -1;         // SBK_KERNEL_*; -1: automatic
     std::map<std::tuple<int, int, int, int, int, int, int>, ItemList> lists;
 
+    // persistent attempts: grid-barrier counters and the timeout flag
+    unsigned *d_bar = nullptr;
+    int *d_timeout = nullptr;
+    bool pers_launched = false;   // a persistent launch ran since the states were last read
+    long long pers_count = 0;     // persistent launches since the last reset_stats
+    int coop = 0;                 // the device supports cooperative launches
+
     // graph of one attempt
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -616,6 +623,10 @@ int finalize(sbk_engine *e)
     e->dev.nblocks = (int)blocks.size();
     HIP_TRY(hipMalloc(&e->d_params, sizeof(sbk::CloudParams) * e->clouds.size()));
     HIP_TRY(hipMalloc(&e->d_state, sizeof(sbk::CloudState) * e->clouds.size()));
+    HIP_TRY(hipMalloc(&e->d_bar, sizeof(unsigned) * 2));
+    HIP_TRY(hipMalloc(&e->d_timeout, sizeof(int)));
+    HIP_TRY(hipMemsetAsync(e->d_bar, 0, sizeof(unsigned) * 2, e->stream));
+    HIP_TRY(hipMemsetAsync(e->d_timeout, 0, sizeof(int), e->stream));
     HIP_TRY(hipHostMalloc(&e->h_state, sizeof(sbk::CloudState) * e->clouds.size(),
                           hipHostMallocDefault));
 
@@ -785,8 +796,89 @@ int enqueue_attempt(sbk_engine *e)
     return 0;
 }
 
+// Grid of the persistent attempt kernel for this engine (0: none): enough
+// work-groups for the largest phase, at most what is resident at once.
+int persistent_grid(sbk_engine *e, const ItemList &L)
+{
+    const int precision = e->cfg.precision;
+    const int tpt = precision == SBK_FP64_EXACT ? 1 : targets_per_thread(e, precision);
+    if (precision != SBK_FP64_EXACT && e->groups != 4) return 0;
+    const int per_cu = sbk::persistent_blocks_per_cu(kp(precision), tpt, e->unit_charges ? 1 : 0,
+                                                     local_coords(e, precision));
+    if (per_cu <= 0) return 0;
+    const long long cap = (long long)per_cu * e->cus;
+    const long long want = std::max<long long>({(long long)L.nitems, (long long)e->dev.nblocks,
+                                                (long long)e->clouds.size(), 1LL});
+    return (int)std::min(cap, want);
+}
+
+// Whether the next attempts run as one persistent launch (persistent = 1;
+// a shape without such a kernel is an error, not a silent fallback).
+int use_persistent(sbk_engine *e, bool *use)
+{
+    *use = false;
+    if (e->cfg.persistent != 1 || e->cfg.timing) return 0;
+    const int precision = e->cfg.precision;
+    const int tpt = precision == SBK_FP64_EXACT ? 1 : targets_per_thread(e, precision);
+    bool ok = e->coop != 0;
+    if (precision != SBK_FP64_EXACT && (e->groups != 4 || (tpt != 1 && tpt != 2))) ok = false;
+    if (ok)
+        ok = sbk::persistent_blocks_per_cu(kp(precision), tpt, e->unit_charges ? 1 : 0,
+                                           local_coords(e, precision)) > 0;
+    if (!ok)
+        return fail(SBK_EINVAL,
+                    "persistent = 1: no persistent attempt kernel for this shape (needs "
+                    "cooperative launch, 4 source groups and 1 or 2 targets per thread; "
+                    "have %d groups, %d targets per thread)", e->groups, tpt);
+    *use = ok;
+    return 0;
+}
+
+// `count` attempts as one cooperative launch (see attempts_kernel).
+int run_persistent(sbk_engine *e, int count)
+{
+    ItemList *L = nullptr;
+    int rc = get_list(e, e->cfg.precision, true, &L);
+    if (rc) return rc;
+    if ((rc = check_partials(e, L))) return rc;
+    bool any_coulomb = false;
+    for (auto &c : e->clouds) any_coulomb |= (c.f.coulomb && c.n > 0);
+    const int precision = e->cfg.precision;
+    sbk::PersistArgs p{};
+    p.d = e->dev;
+    p.d.slices = L->d_slices;
+    p.a = coulomb_args(e, *L, e->dev.P0, true);
+    p.a.local_coords = local_coords(e, precision);
+    if (precision == SBK_FP64 || precision == SBK_FP64_RSQ) p.a.eps_d = 1e-40;   // bin/force.cpp:136
+    p.bar = e->d_bar;
+    p.timeout = e->d_timeout;
+    p.spin_limit = e->cfg.persistent_spin > 0 ? (unsigned)e->cfg.persistent_spin : 1u << 22;
+    p.attempts = count;
+    p.any_coulomb = any_coulomb && L->nitems > 0;
+    const int grid = persistent_grid(e, *L);
+    if (grid <= 0) return fail(SBK_ESTATE, "no persistent attempt kernel for this launch shape");
+    HIP_TRY(hipMemsetAsync(e->d_bar, 0, sizeof(unsigned) * 2, e->stream));
+    const int tpt = precision == SBK_FP64_EXACT ? 1 : targets_per_thread(e, precision);
+    const hipError_t err =
+        (hipError_t)sbk::launch_attempts_persistent(kp(precision), tpt, p, grid, e->stream);
+    if (err != hipSuccess) {
+        (void)hipGetLastError();   // reported here
+        // a refused launch enqueued nothing; cooperative queues are a limited
+        // resource of the device, which another process may hold
+        return fail(SBK_EHIP, "persistent attempt kernel launch failed (grid %d): %s", grid,
+                    hipGetErrorString(err));
+    }
+    e->pers_launched = true;
+    e->pers_count++;
+    return 0;
+}
+
 int run_attempts(sbk_engine *e, int count)
 {
+    if (count <= 0) return 0;
+    bool pers = false;
+    if (int rc = use_persistent(e, &pers)) return rc;
+    if (pers) return run_persistent(e, count);
     if (e->cfg.use_graph && !e->cfg.timing) {
         if (!e->graph_exec) {
             // build the list before capture: allocation is not allowed while capturing
@@ -819,7 +911,17 @@ int read_states(sbk_engine *e)
 {
     HIP_TRY(hipMemcpyAsync(e->h_state, e->d_state, sizeof(sbk::CloudState) * e->clouds.size(),
                            hipMemcpyDeviceToHost, e->stream));
+    int timeout = 0;
+    if (e->pers_launched)
+        HIP_TRY(hipMemcpyAsync(&timeout, e->d_timeout, sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
+    e->pers_launched = false;
+    if (timeout) {
+        (void)hipMemsetAsync(e->d_timeout, 0, sizeof(int), e->stream);
+        (void)hipStreamSynchronize(e->stream);
+        return fail(SBK_EHIP, "a grid barrier of the persistent attempt kernel timed out; the "
+                              "state is undefined (use persistent = -1)");
+    }
     return harvest_events(e);
 }
 
@@ -967,6 +1069,10 @@ int sbk_create(const sbk_config *cfg, sbk_engine **out)
     if (!cfg || !out) return fail(SBK_EINVAL, "null argument");
     if (cfg->precision < SBK_FP64_EXACT || cfg->precision > SBK_FP64_RSQ)
         return fail(SBK_EINVAL, "unknown precision %d", cfg->precision);
+    if (cfg->persistent < 0 || cfg->persistent > 1)
+        return fail(SBK_EINVAL, "persistent must be 0 or 1 (got %d)", cfg->persistent);
+    if (cfg->persistent_spin < 0)
+        return fail(SBK_EINVAL, "persistent_spin must be >= 0 (got %d)", cfg->persistent_spin);
     if (cfg->time_mode != SBK_TIME_FROZEN && cfg->time_mode != SBK_TIME_STAGE)
         return fail(SBK_EINVAL, "unknown time mode %d", cfg->time_mode);
     if (!(cfg->h0 > 0.0) || !(cfg->rtol >= 0.0) || !(cfg->atol >= 0.0) ||
@@ -980,6 +1086,7 @@ int sbk_create(const sbk_config *cfg, sbk_engine **out)
     hipDeviceProp_t p;
     HIP_TRY(hipGetDeviceProperties(&p, e->device));
     e->cus = std::max(1, p.multiProcessorCount);
+    e->coop = p.cooperativeLaunch;
     HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     *out = e.release();
     return 0;
@@ -1006,6 +1113,8 @@ int sbk_destroy(sbk_engine *e)
     (void)hipFree(e->d_swift);
     (void)hipFree(e->d_xc);
     (void)hipFree(e->d_xc_count);
+    (void)hipFree(e->d_bar);
+    (void)hipFree(e->d_timeout);
     if (e->h_state) (void)hipHostFree(e->h_state);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -1345,6 +1454,7 @@ int sbk_get_stats(sbk_engine *e, sbk_stats *out)
     out->coulomb_ms = e->coulomb_ms;
     out->coulomb_launches = e->coulomb_launches;
     out->reorders = e->reorders - e->reorders_at_reset;
+    out->persistent_launches = e->pers_count;
     for (size_t i = 0; i < e->clouds.size(); ++i) {
         const double n = e->clouds[i].n;
         const long long att = e->h_state[i].attempts - e->attempts_at_reset[i];
@@ -1371,6 +1481,7 @@ int sbk_reset_stats(sbk_engine *e)
     }
     e->coulomb_ms = 0.0;
     e->coulomb_launches = 0;
+    e->pers_count = 0;
     e->reorders_at_reset = e->reorders;
     return 0;
 }

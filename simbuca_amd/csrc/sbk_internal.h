@@ -137,6 +137,23 @@ struct EngineDev {
     int pad_xc;
 };
 
+// Persistent attempts: one cooperative launch of gridDim work-groups runs up
+// to `attempts` whole attempts (7 x (Coulomb items -> stage), controller,
+// commit), separated by grid barriers, and stops early once no cloud is
+// active.  The barrier counters live in bar[0] (arrivals) and bar[1]
+// (generation); a wait that spins past spin_limit sets *timeout, after which
+// every barrier returns at once and every work-group leaves.
+struct PersistArgs {
+    EngineDev d;
+    CoulombArgs a;                // items, weights, prefactors, partials (positions set per stage)
+    unsigned *bar;
+    int *timeout;
+    unsigned spin_limit;
+    int attempts;
+    int any_coulomb;
+    int pad;
+};
+
 // sets the message sbk_last_error() returns; returns code (sbk_engine.cpp)
 int record_error(int code, const char *msg);
 
@@ -163,6 +180,15 @@ int launch_commit(const EngineDev &d, void *stream);
 // set_end: 0 keep t_end, 1 t_end = t + duration, 2 t_end = duration (absolute)
 int launch_begin(const EngineDev &d, double duration, int budget, int set_end, void *stream);
 int launch_eval_derivs(const EngineDev &d, int with_fields, void *stream);
+// Persistent attempts (cooperative launch) for precisions with 1 or 2 targets per
+// thread and 4 source groups (FP64-exact: 1 and 1); mixed precisions use the
+// streamed Coulomb item.  Returns the launch's hipError_t (hipErrorInvalidValue
+// when no such kernel exists); a refused launch has enqueued nothing.
+int launch_attempts_persistent(int precision, int targets_per_thread, const PersistArgs &p,
+                               int grid, void *stream);
+// resident work-groups per CU of that kernel (0: none)
+int persistent_blocks_per_cu(int precision, int targets_per_thread, int unit_weights,
+                             int local_coords);
 // out[g] = in[src[g]] for g < n
 int launch_gather(const int *src, const double *in, double *out, int n, void *stream);
 int launch_gather_f32(const int *src, const float *in, float *out, int n, void *stream);

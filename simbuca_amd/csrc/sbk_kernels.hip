@@ -38,10 +38,10 @@ __device__ __forceinline__ int xcd_item(int b, int nb)
 // ((r*invDist^3)*k_g)*q_s with r = x_s - x_g, invDist = 1/sqrt(|r|^2)
 // (bin/force.cpp:158-170).  One thread per target, no source split.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void coulomb_fp64_exact(CoulombArgs a)
+__device__ __forceinline__ void coulomb_fp64_exact_item(const CoulombArgs &a, int item)
 {
     __shared__ double lx[kBlock], ly[kBlock], lz[kBlock], lq[kBlock];
-    const CoulombItem it = a.items[xcd_item(blockIdx.x, gridDim.x)];
+    const CoulombItem it = a.items[item];
     if (a.state && (!a.state[it.cloud].active || (a.skip_valid_k1 && a.state[it.cloud].k1_valid)))
         return;
     const int tid = threadIdx.x;
@@ -81,6 +81,11 @@ __global__ __launch_bounds__(kBlock) void coulomb_fp64_exact(CoulombArgs a)
         o[a.stride] = fy;
         o[2 * a.stride] = fz;
     }
+}
+
+__global__ __launch_bounds__(kBlock) void coulomb_fp64_exact(CoulombArgs a)
+{
+    coulomb_fp64_exact_item(a, xcd_item(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -149,7 +154,7 @@ __device__ __forceinline__ double rsq3_f64(double x)
 }
 
 template <int R, int MODE, bool UNIT, int Q, bool LOC>
-__global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
+__device__ __forceinline__ void coulomb_tiled_item(const CoulombArgs &a, int item)
 {
     constexpr bool F64 = MODE == KP_FP64 || MODE == KP_FP64_RSQ;
     static_assert(!LOC || !F64, "local coordinates are a mixed-precision variant");
@@ -161,7 +166,7 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
     __shared__ float4 lf[LOC ? kBlock : 1];                 // LOC: (x_s - o, weight)
     __shared__ int lfar[LOC ? kBlock / kSubTile : 1];       // LOC: far sources per sub-tile
 
-    const CoulombItem it = a.items[xcd_item(blockIdx.x, gridDim.x)];
+    const CoulombItem it = a.items[item];
     if (a.state && (!a.state[it.cloud].active || (a.skip_valid_k1 && a.state[it.cloud].k1_valid)))
         return;
     const int tid = threadIdx.x;
@@ -411,6 +416,12 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
     }
 }
 
+template <int R, int MODE, bool UNIT, int Q, bool LOC>
+__global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
+{
+    coulomb_tiled_item<R, MODE, UNIT, Q, LOC>(a, xcd_item(blockIdx.x, gridDim.x));
+}
+
 // ---------------------------------------------------------------------------
 // Coulomb direct sum, streamed (mixed modes).  Same work items, targets,
 // pair arithmetic, local coordinates and accumulation as coulomb_tiled with
@@ -425,7 +436,7 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
 // the global-load latency of the next sub-tile overlaps the sums.
 // ---------------------------------------------------------------------------
 template <int R, int MODE, bool UNIT, bool LOC>
-__global__ __launch_bounds__(kBlock) void coulomb_stream(CoulombArgs a)
+__device__ __forceinline__ void coulomb_stream_item(const CoulombArgs &a, int item)
 {
     static_assert(MODE == KP_MIXED || MODE == KP_MIXED_KAHAN, "a mixed-precision kernel");
     constexpr int NW = kBlock / 64;      // waves per work-group: 4 source streams
@@ -433,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void coulomb_stream(CoulombArgs a)
     __shared__ double lx[NW * W], ly[NW * W], lz[NW * W];   // FP64 sources (near path); reused
     __shared__ float4 lf[NW * W];                           // (offset or 0, weight)
 
-    const CoulombItem it = a.items[xcd_item(blockIdx.x, gridDim.x)];
+    const CoulombItem it = a.items[item];
     // uniform over the work-group (one item), so either every wave reaches
     // the closing barrier or none does
     if (a.state && (!a.state[it.cloud].active || (a.skip_valid_k1 && a.state[it.cloud].k1_valid)))
@@ -658,6 +669,12 @@ __global__ __launch_bounds__(kBlock) void coulomb_stream(CoulombArgs a)
             o[2 * a.stride] = k * Az[r];
         }
     }
+}
+
+template <int R, int MODE, bool UNIT, bool LOC>
+__global__ __launch_bounds__(kBlock) void coulomb_stream(CoulombArgs a)
+{
+    coulomb_stream_item<R, MODE, UNIT, LOC>(a, xcd_item(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -894,9 +911,9 @@ __device__ __forceinline__ void coulomb_total(const EngineDev &d, const CloudPar
 // state, then the next stage state (bin/ode.cpp:103-175), or for S == 7 the
 // new state and its scaled error (bin/ode.cpp:177-205).
 template <int S>
-__global__ __launch_bounds__(256) void stage_kernel(EngineDev d)
+__device__ __forceinline__ void stage_block(const EngineDev &d, int block)
 {
-    const IonBlock blk = d.blocks[blockIdx.x];
+    const IonBlock blk = d.blocks[block];
     const int c = blk.cloud;
     const CloudState &cs = d.state[c];
     if (!cs.active) return;                       // uniform over the block
@@ -969,17 +986,22 @@ __global__ __launch_bounds__(256) void stage_kernel(EngineDev d)
             if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
             __syncthreads();
         }
-        if (threadIdx.x == 0) d.ERRB[blockIdx.x] = red[0];
+        if (threadIdx.x == 0) d.ERRB[block] = red[0];
     }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void stage_kernel(EngineDev d)
+{
+    stage_block<S>(d, blockIdx.x);
 }
 
 // Per-cloud error norm (bin/ode.cpp:215), controller (bin/ode.cpp:426-457)
 // and the step bookkeeping of bin/ode.cpp:470-493.  One work-group per cloud;
 // the reduction order is fixed, so results are run-to-run reproducible.
-__global__ __launch_bounds__(256) void controller_kernel(EngineDev d)
+__device__ __forceinline__ void controller_cloud(const EngineDev &d, int c)
 {
     __shared__ double red[256];
-    const int c = blockIdx.x;
     CloudState &cs = d.state[c];
     const int active = cs.active;
     if (!active) {
@@ -1045,14 +1067,19 @@ __global__ __launch_bounds__(256) void controller_kernel(EngineDev d)
     }
 }
 
+__global__ __launch_bounds__(256) void controller_kernel(EngineDev d)
+{
+    controller_cloud(d, blockIdx.x);
+}
+
 // pos <- pos2, vel <- vel2 for clouds whose attempt was accepted
 // (bin/ode.cpp:488-489).
 // With a crossing plane, an ion whose x lies strictly on one side of it
 // before the step and strictly on the other after it (bin/ion_fly.cpp:91-93,
 // 110-114) is appended to the crossing list with its new state.
-__global__ __launch_bounds__(256) void commit_kernel(EngineDev d)
+__device__ __forceinline__ void commit_block(const EngineDev &d, int block)
 {
-    const IonBlock blk = d.blocks[blockIdx.x];
+    const IonBlock blk = d.blocks[block];
     const CloudState &cs = d.state[blk.cloud];
     if (!cs.accepted_last) return;
     const long long g = blk.start + threadIdx.x;
@@ -1083,6 +1110,11 @@ __global__ __launch_bounds__(256) void commit_kernel(EngineDev d)
         d.V0[k][g] = d.KV[7][k][g];
         if (fsal) d.KA[1][k][g] = d.KA[7][k][g];
     }
+}
+
+__global__ __launch_bounds__(256) void commit_kernel(EngineDev d)
+{
+    commit_block(d, blockIdx.x);
 }
 
 __global__ void begin_kernel(EngineDev d, double duration, int budget, int set_end)
@@ -1130,6 +1162,146 @@ __global__ __launch_bounds__(256) void gather_kernel(const int *src, const T *in
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n) out[g] = in[src[g]];
+}
+
+// ---------------------------------------------------------------------------
+// Persistent attempts.  A cooperative launch (every work-group resident at
+// once) runs whole attempts with a grid barrier between the phases that read
+// what another work-group wrote: Coulomb items -> stage s (7 times), then the
+// controller and the commit.  Each phase deals its items (Coulomb work items,
+// ion blocks, clouds) to the work-groups round-robin and calls the same
+// device functions the one-phase kernels call, so results are the graph
+// path's bit for bit.  It replaces the 16 launches of an attempt with one
+// launch per batch, but on MI355X a grid barrier (about 2 us at 16
+// work-groups, 13 us at 512: the cross-XCD release/acquire and the arrivals
+// on one address) costs more than a graph's kernel boundary (1.7 us), so the
+// engine uses it only on request.
+//
+// The barrier: thread 0 of each work-group reads the generation, adds itself
+// to the arrivals (agent scope, release) and either, as the last one,
+// acquires, resets the arrivals and bumps the generation (release), or polls
+// the generation with relaxed loads and acquires once it moved.  Every wait
+// is bounded: past spin_limit it sets the timeout flag, which makes every
+// later barrier return false at once, so all work-groups leave and the
+// kernel ends; the host reports the error.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool grid_barrier(const PersistArgs &p)
+{
+    __shared__ int s_ok;
+    __syncthreads();   // the work-group's writes of this phase are done
+    if (threadIdx.x == 0) {
+        int ok = __hip_atomic_load(p.timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+        if (ok) {
+            // the generation cannot move before this work-group arrives; the
+            // release of the arrival orders this load before it
+            const unsigned gen =
+                __hip_atomic_load(p.bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned arrived =
+                __hip_atomic_fetch_add(p.bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+            if (arrived == gridDim.x) {
+                // every other arrival's release, then the waiters' release
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                __hip_atomic_store(p.bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(p.bar + 1, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                // relaxed polls: an acquire per poll would invalidate the caches
+                // the work-groups still computing are using
+                unsigned k = 0;
+                while (__hip_atomic_load(p.bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                       gen) {
+                    if (++k > p.spin_limit ||
+                        __hip_atomic_load(p.timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        __hip_atomic_store(p.timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ok = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+                // one acquire: every work-group's writes of the phase are visible
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// KIND 0: FP64-exact items; 1: tiled items (Q = 4); 2: streamed items
+template <int R, int MODE, bool UNIT, bool LOC, int KIND>
+__device__ __forceinline__ void persistent_item(const CoulombArgs &a, int k)
+{
+    if constexpr (KIND == 0)
+        coulomb_fp64_exact_item(a, k);
+    else if constexpr (KIND == 1)
+        coulomb_tiled_item<R, MODE, UNIT, 4, LOC>(a, k);
+    else
+        coulomb_stream_item<R, MODE, UNIT, LOC>(a, k);
+}
+
+// Coulomb items of stage S, barrier, stage S, barrier; false once a barrier failed
+template <int S, int R, int MODE, bool UNIT, bool LOC, int KIND>
+__device__ __forceinline__ bool persistent_stage(const PersistArgs &p)
+{
+    const EngineDev &d = p.d;
+    if (p.any_coulomb) {
+        CoulombArgs a = p.a;
+        double *const *pos = (S == 1) ? d.P0 : d.PS;
+        a.tx = a.sx = pos[0];
+        a.ty = a.sy = pos[1];
+        a.tz = a.sz = pos[2];
+        a.skip_valid_k1 = (S == 1);
+        for (int k = blockIdx.x; k < a.nitems; k += gridDim.x) {
+            __syncthreads();   // the previous item's LDS is free
+            persistent_item<R, MODE, UNIT, LOC, KIND>(a, k);
+        }
+        if (!grid_barrier(p)) return false;
+    }
+    for (int b = blockIdx.x; b < d.nblocks; b += gridDim.x) {
+        __syncthreads();
+        stage_block<S>(d, b);
+    }
+    return grid_barrier(p);
+}
+
+template <int R, int MODE, bool UNIT, bool LOC, int KIND>
+__global__ __launch_bounds__(kBlock) void attempts_kernel(PersistArgs p)
+{
+    const EngineDev &d = p.d;
+    __shared__ int s_any;
+    for (int att = 0; att < p.attempts; ++att) {
+        if (!persistent_stage<1, R, MODE, UNIT, LOC, KIND>(p)) return;
+        if (!persistent_stage<2, R, MODE, UNIT, LOC, KIND>(p)) return;
+        if (!persistent_stage<3, R, MODE, UNIT, LOC, KIND>(p)) return;
+        if (!persistent_stage<4, R, MODE, UNIT, LOC, KIND>(p)) return;
+        if (!persistent_stage<5, R, MODE, UNIT, LOC, KIND>(p)) return;
+        if (!persistent_stage<6, R, MODE, UNIT, LOC, KIND>(p)) return;
+        if (!persistent_stage<7, R, MODE, UNIT, LOC, KIND>(p)) return;
+        for (int c = blockIdx.x; c < d.nclouds; c += gridDim.x) {
+            __syncthreads();
+            controller_cloud(d, c);
+        }
+        if (!grid_barrier(p)) return;
+        for (int b = blockIdx.x; b < d.nblocks; b += gridDim.x) {
+            __syncthreads();
+            commit_block(d, b);
+        }
+        if (!grid_barrier(p)) return;
+        // another attempt only while some cloud is active (the same answer in
+        // every work-group: the states were last written before this barrier)
+        if (threadIdx.x == 0) s_any = 0;
+        __syncthreads();
+        for (int c = threadIdx.x; c < d.nclouds; c += blockDim.x)
+            if (d.state[c].active) s_any = 1;
+        __syncthreads();
+        if (!s_any) {
+            // what the remaining attempts' controllers would write for
+            // inactive clouds; nothing else reads the states before the end
+            if (att + 1 < p.attempts && threadIdx.x == 0)
+                for (int c = blockIdx.x; c < d.nclouds; c += gridDim.x) d.state[c].accepted_last = 0;
+            return;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1288,6 +1460,56 @@ int launch_eval_derivs(const EngineDev &d, int with_fields, void *stream)
     hipLaunchKernelGGL(eval_derivs_kernel, dim3((unsigned)d.nblocks), dim3(256), 0,
                        (hipStream_t)stream, d, with_fields);
     return check_launch();
+}
+
+template <int R>
+static const void *persistent_fn_r(int precision, int unit, int loc)
+{
+#define SBK_PERS(M, U, L, K) reinterpret_cast<const void *>(&attempts_kernel<R, M, U, L, K>)
+    switch (precision) {
+    case KP_FP64: return loc ? nullptr : SBK_PERS(KP_FP64, false, false, 1);
+    case KP_FP64_RSQ: return loc ? nullptr : SBK_PERS(KP_FP64_RSQ, false, false, 1);
+    case KP_MIXED:
+        return unit ? (loc ? SBK_PERS(KP_MIXED, true, true, 2) : SBK_PERS(KP_MIXED, true, false, 2))
+                    : (loc ? SBK_PERS(KP_MIXED, false, true, 2) : SBK_PERS(KP_MIXED, false, false, 2));
+    case KP_MIXED_KAHAN:
+        return unit ? (loc ? SBK_PERS(KP_MIXED_KAHAN, true, true, 2)
+                           : SBK_PERS(KP_MIXED_KAHAN, true, false, 2))
+                    : (loc ? SBK_PERS(KP_MIXED_KAHAN, false, true, 2)
+                           : SBK_PERS(KP_MIXED_KAHAN, false, false, 2));
+    default: return nullptr;
+    }
+#undef SBK_PERS
+}
+
+static const void *persistent_fn(int precision, int tpt, int unit, int loc)
+{
+    if (precision == KP_FP64_EXACT)
+        return (tpt == 1 && !loc)
+                   ? reinterpret_cast<const void *>(&attempts_kernel<1, KP_FP64_EXACT, false, false, 0>)
+                   : nullptr;
+    if (tpt == 1) return persistent_fn_r<1>(precision, unit, loc);
+    if (tpt == 2) return persistent_fn_r<2>(precision, unit, loc);
+    return nullptr;
+}
+
+int persistent_blocks_per_cu(int precision, int tpt, int unit, int loc)
+{
+    const void *f = persistent_fn(precision, tpt, unit, loc);
+    if (!f) return 0;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kBlock, 0) != hipSuccess) return 0;
+    return nb;
+}
+
+int launch_attempts_persistent(int precision, int tpt, const PersistArgs &p, int grid, void *stream)
+{
+    const void *f = persistent_fn(precision, tpt, p.a.unit_weights, p.a.local_coords);
+    if (!f || grid <= 0) return (int)hipErrorInvalidValue;
+    void *args[] = {const_cast<PersistArgs *>(&p)};
+    // the launch's own status: a refused cooperative launch enqueues nothing
+    return (int)hipLaunchCooperativeKernel(f, dim3(grid), dim3(kBlock), args, 0,
+                                           (hipStream_t)stream);
 }
 
 }  // namespace sbk
