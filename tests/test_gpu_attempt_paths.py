@@ -1,12 +1,13 @@
-"""Batches of attempts run as one persistent (cooperative) launch, against
-the same attempts replayed from the per-attempt graph.
+"""Persistent attempts (persistent = 1: a batch of attempts as one
+cooperative launch with grid barriers between the phases) against the
+graph replay of the one-phase kernels, and the separate launches (no
+graph) against both.
 
-The persistent kernel calls, phase by phase, the device functions of the
-one-phase kernels (Coulomb work items, stage blocks, the controller, the
-commit), with a grid barrier between phases.  Every sum keeps its order, so
-states, statuses and statistics must agree bit for bit with the graph path
-of the same launch shape: positions, velocities, h, errold, attempt and step
-counts, accepted_last.  The cases cover every precision, 1 and 2 targets per
+The persistent kernel calls the device functions of the one-phase kernels
+(Coulomb work items, stage blocks, the controller, the commit) and keeps
+every sum's order, so states, statuses and statistics must agree bit for
+bit with the graph path of the same launch shape: positions, velocities, h,
+errold, attempt and step counts, accepted_last.  The cases cover every precision, 1 and 2 targets per
 thread, unit and mixed charges, local coordinates on and off, rejected
 attempts (a first step far too long), clouds that finish in the middle of a
 batch, a cloud with its Coulomb switch off, an empty cloud, a crossing
@@ -31,12 +32,15 @@ def _cloud(rng, n, sigma=5e-4, unit=False, amu=133.0):
     return pos, vel, m, ch.astype(np.int32)
 
 
-def _run(clouds, fields, precision, tpt, persistent, local=-1, duration=3e-7, h0=4e-7,
-         crossing=None, adaptive=True, batches=None):
+PATHS = {"graph": dict(), "persistent": dict(persistent=1)}
+
+
+def _run(clouds, fields, precision, tpt, path, local=-1, duration=3e-7, h0=4e-7,
+         crossing=None, adaptive=True, batches=None, use_graph=True):
     """Advances every cloud by `duration` and returns states, statuses, stats
     and crossings."""
-    with S.Engine(precision=precision, persistent=persistent, rtol=1e-9, atol=1e-9, h0=h0,
-                  time_mode=S.TIME_STAGE, adaptive=adaptive) as e:
+    with S.Engine(precision=precision, rtol=1e-9, atol=1e-9, h0=h0, time_mode=S.TIME_STAGE,
+                  adaptive=adaptive, use_graph=use_graph, **PATHS[path]) as e:
         for (pos, vel, m, ch), f in zip(clouds, fields):
             e.add_cloud(pos, vel, m, ch, f)
         if precision != "fp64_exact":
@@ -75,21 +79,24 @@ CASES = [("fp64_exact", 1, -1, False), ("fp64", 1, -1, False), ("fp64", 2, -1, F
          ("mixed", 1, 1, True), ("mixed_kahan", 2, 1, True), ("mixed_kahan", 1, 0, False)]
 
 
-@pytest.mark.parametrize("precision,tpt,local,unit", CASES)
-def test_persistent_equals_graph_single_cloud(rng, precision, tpt, local, unit):
+def _used(path, stats):
+    return (stats["persistent_launches"] > 0) == (path == "persistent")
+
+
+@pytest.mark.parametrize("path,precision,tpt,local,unit", [("persistent",) + c for c in CASES])
+def test_path_equals_graph_single_cloud(rng, path, precision, tpt, local, unit):
     clouds = [_cloud(rng, 700, unit=unit)]
     fields = [S.default_fields(**PENNING)]
-    print(f"single cloud {precision} tpt={tpt} local={local} unit={unit}", flush=True)
-    g = _run(clouds, fields, precision, tpt, 0, local)
-    p = _run(clouds, fields, precision, tpt, 1, local)
-    assert g[2]["persistent_launches"] == 0
-    assert p[2]["persistent_launches"] > 0
+    print(f"single cloud {path} {precision} tpt={tpt} local={local} unit={unit}", flush=True)
+    g = _run(clouds, fields, precision, tpt, "graph", local)
+    p = _run(clouds, fields, precision, tpt, path, local)
+    assert _used("graph", g[2]) and _used(path, p[2])
     assert p[1][0]["steps"] > 10 and p[1][0]["attempts"] > p[1][0]["steps"]   # rejects happened
     _same(g, p)
 
 
-@pytest.mark.parametrize("precision,tpt,local,unit", CASES[::2])
-def test_persistent_equals_graph_ensemble(rng, precision, tpt, local, unit):
+@pytest.mark.parametrize("path,precision,tpt,local,unit", [("persistent",) + c for c in CASES[::2]])
+def test_path_equals_graph_ensemble(rng, path, precision, tpt, local, unit):
     """Clouds of different sizes, species and fields: they take different
     numbers of attempts, so some finish in the middle of a batch; one cloud
     has no Coulomb term and one is empty."""
@@ -99,28 +106,29 @@ def test_persistent_equals_graph_ensemble(rng, precision, tpt, local, unit):
                                  [133.0, 85.0, 39.0, 133.0, 133.0, 85.0])]
     fields = [S.default_fields(**dict(PENNING, Bz=b)) for b in (7.0, 5.0, 7.0, 7.0, 3.0, 7.0)]
     fields[5] = S.default_fields(**dict(PENNING, coulomb=0))
-    print(f"ensemble {precision} tpt={tpt} local={local} unit={unit}", flush=True)
-    g = _run(clouds, fields, precision, tpt, 0, local, crossing=0.0)
-    p = _run(clouds, fields, precision, tpt, 1, local, crossing=0.0)
-    assert p[2]["persistent_launches"] > 0
+    print(f"ensemble {path} {precision} tpt={tpt} local={local} unit={unit}", flush=True)
+    g = _run(clouds, fields, precision, tpt, "graph", local, crossing=0.0)
+    p = _run(clouds, fields, precision, tpt, path, local, crossing=0.0)
+    assert _used("graph", g[2]) and _used(path, p[2])
     steps = [st["steps"] for st in p[1]]
     assert len(set(steps[:3] + steps[4:])) > 1            # the clouds took different step counts
     assert p[3]["ion"].size > 0                            # ions crossed x = 0
     _same(g, p)
 
 
-def test_persistent_batches_and_fixed_steps(rng):
+def test_paths_batches_and_fixed_steps(rng):
     """Open-ended batches of odd sizes (every attempt runs, none ends early)
-    with fixed steps, against the graph path."""
+    with fixed steps: persistent and separate launches against the graph."""
     clouds = [_cloud(rng, 500), _cloud(rng, 130)]
     fields = [S.default_fields(**PENNING)] * 2
-    runs = []
-    for pers in (0, 1):
-        runs.append(_run(clouds, fields, "mixed", 1, pers, 1, h0=2e-9, adaptive=False,
-                         batches=[1, 3, 7, 2]))
-    _same(runs[0], runs[1])
-    assert runs[1][1][0]["attempts"] == 13 and runs[1][1][0]["steps"] == 13
-    assert runs[1][2]["persistent_launches"] == 4
+    runs = {}
+    for path, graph in (("graph", True), ("persistent", True), ("graph", False)):
+        runs[path, graph] = _run(clouds, fields, "mixed", 1, path, 1, h0=2e-9, adaptive=False,
+                                 batches=[1, 3, 7, 2], use_graph=graph)
+    for key in runs:
+        _same(runs["graph", True], runs[key])
+    assert runs["graph", True][1][0]["attempts"] == 13 and runs["graph", True][1][0]["steps"] == 13
+    assert runs["persistent", True][2]["persistent_launches"] == 4
 
 
 def test_persistent_is_opt_in(rng):
@@ -143,7 +151,7 @@ def test_persistent_rejects_shape_without_kernel(rng):
     clouds = [_cloud(rng, 300)]
     with S.Engine(precision="mixed", persistent=1) as e:
         e.add_cloud(*clouds[0], S.default_fields(**PENNING))
-        e.set_launch(4, 4, 0, -1)
+        e.set_launch(4, 4, 0, -1)   # 4 targets per thread: no persistent kernel
         with pytest.raises(S.SimbucaError, match="persistent"):
             e.advance(1e-8)
 
