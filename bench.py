@@ -418,9 +418,10 @@ def other_workload(S, w, args, precision):
                               "frac": tf / peak}
         e.close()
         log(f"{p}: {res[p]}")
-        if res["pairs_per_attempt"] < 7e8:
-            # launch-bound sizes: the whole attempt, not the Coulomb kernel, sets the rate
-            res[p]["replay"] = replay(S, w, p, max(args.steps, 400), args.warmup, tune)
+        if res["pairs_per_attempt"] < 1e10:
+            # the graph replay against persistent attempts, about 1 s of attempts
+            k = max(args.steps, min(400, int(2e11 / res["pairs_per_attempt"])))
+            res[p]["replay"] = replay(S, w, p, k, args.warmup, tune)
             log(f"{p} replay: {res[p]['replay']}")
         if args.to_end:
             dte, ste = run_to_end(S, w, p, tune)

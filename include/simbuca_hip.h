@@ -205,7 +205,7 @@ typedef struct sbk_config {
                            separate launches.  On MI355X a grid barrier (2 us at 16 work-groups,
                            13 us at 512) costs more than a graph's kernel boundary (1.7 us), so
                            the graph is faster at every size measured.  Needs 4 source groups
-                           and 1 or 2 targets per thread (SBK_EINVAL otherwise) and a cooperative
+                           and 1, 2 or 4 targets per thread (SBK_EINVAL otherwise) and a cooperative
                            queue, which the device may refuse while another process holds it
                            (SBK_EHIP); timing = 1 uses separate launches. */
     int    persistent_spin; /* polls of a grid-barrier wait before the launch gives up and

@@ -821,14 +821,15 @@ int use_persistent(sbk_engine *e, bool *use)
     const int precision = e->cfg.precision;
     const int tpt = precision == SBK_FP64_EXACT ? 1 : targets_per_thread(e, precision);
     bool ok = e->coop != 0;
-    if (precision != SBK_FP64_EXACT && (e->groups != 4 || (tpt != 1 && tpt != 2))) ok = false;
+    if (precision != SBK_FP64_EXACT && (e->groups != 4 || (tpt != 1 && tpt != 2 && tpt != 4)))
+        ok = false;
     if (ok)
         ok = sbk::persistent_blocks_per_cu(kp(precision), tpt, e->unit_charges ? 1 : 0,
                                            local_coords(e, precision)) > 0;
     if (!ok)
         return fail(SBK_EINVAL,
                     "persistent = 1: no persistent attempt kernel for this shape (needs "
-                    "cooperative launch, 4 source groups and 1 or 2 targets per thread; "
+                    "cooperative launch, 4 source groups and 1, 2 or 4 targets per thread; "
                     "have %d groups, %d targets per thread)", e->groups, tpt);
     *use = ok;
     return 0;

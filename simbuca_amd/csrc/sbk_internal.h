@@ -180,8 +180,8 @@ int launch_commit(const EngineDev &d, void *stream);
 // set_end: 0 keep t_end, 1 t_end = t + duration, 2 t_end = duration (absolute)
 int launch_begin(const EngineDev &d, double duration, int budget, int set_end, void *stream);
 int launch_eval_derivs(const EngineDev &d, int with_fields, void *stream);
-// Persistent attempts (cooperative launch) for precisions with 1 or 2 targets per
-// thread and 4 source groups (FP64-exact: 1 and 1); mixed precisions use the
+// Persistent attempts (cooperative launch) for precisions with 1, 2 or 4 targets
+// per thread and 4 source groups (FP64-exact: 1 and 1); mixed precisions use the
 // streamed Coulomb item.  Returns the launch's hipError_t (hipErrorInvalidValue
 // when no such kernel exists); a refused launch has enqueued nothing.
 int launch_attempts_persistent(int precision, int targets_per_thread, const PersistArgs &p,

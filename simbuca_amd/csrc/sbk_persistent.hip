@@ -74,6 +74,7 @@ static const void *persistent_fn(int precision, int tpt, int unit, int loc)
                    : nullptr;
     if (tpt == 1) return persistent_fn_r<1>(precision, unit, loc);
     if (tpt == 2) return persistent_fn_r<2>(precision, unit, loc);
+    if (tpt == 4) return persistent_fn_r<4>(precision, unit, loc);
     return nullptr;
 }
 

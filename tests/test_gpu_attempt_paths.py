@@ -76,7 +76,8 @@ def _same(a, b):
 
 CASES = [("fp64_exact", 1, -1, False), ("fp64", 1, -1, False), ("fp64", 2, -1, False),
          ("fp64_rsq", 2, -1, False), ("mixed", 1, 0, False), ("mixed", 2, 1, False),
-         ("mixed", 1, 1, True), ("mixed_kahan", 2, 1, True), ("mixed_kahan", 1, 0, False)]
+         ("mixed", 1, 1, True), ("mixed_kahan", 2, 1, True), ("mixed_kahan", 1, 0, False),
+         ("fp64", 4, -1, False), ("mixed", 4, 1, True), ("mixed_kahan", 4, 0, False)]
 
 
 def _used(path, stats):
@@ -151,7 +152,7 @@ def test_persistent_rejects_shape_without_kernel(rng):
     clouds = [_cloud(rng, 300)]
     with S.Engine(precision="mixed", persistent=1) as e:
         e.add_cloud(*clouds[0], S.default_fields(**PENNING))
-        e.set_launch(4, 4, 0, -1)   # 4 targets per thread: no persistent kernel
+        e.set_launch(8, 4, 0, -1)   # 8 targets per thread: no persistent kernel
         with pytest.raises(S.SimbucaError, match="persistent"):
             e.advance(1e-8)
 
