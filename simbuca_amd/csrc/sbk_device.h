@@ -110,11 +110,14 @@ __device__ __forceinline__ void coulomb_fp64_exact_item(const CoulombArgs &a, in
 // LOC (mixed modes only): the work-group takes the centre o of its targets'
 // bounding box (half-diagonal h) as origin, and each source also enters LDS
 // as the FP32 offset x_s - o (one FP64 subtraction per source per
-// work-group).  A source is far when |x_s - o| >= (1 + kNearFactor) h; then
-// every separation from it is at least kNearFactor h long, and the FP32
+// work-group).  A source is far when its distance from the targets' bounding
+// box is at least kNearFactor h; then every separation r from it is at least
+// kNearFactor h long, its offset obeys |x_s - o| <= |r| + h, and the FP32
 // difference of two offsets has an error of at most
-// 2^-24 (2 + 2/kNearFactor) |r|, against 2^-24 |r| for the FP64-formed
-// separation.  The wave that stages a 64-source sub-tile stores its far
+// 2^-24 (|x_s - o| + |x_t - o| + |r|) <= 2^-24 (2 + 2/kNearFactor) |r|,
+// against 2^-24 |r| for the FP64-formed separation.  (The box distance marks
+// about half as many sources near as a sphere of radius (1 + kNearFactor) h
+// around o would, with the same bound.)  The wave that stages a 64-source sub-tile stores its far
 // sources first (a ballot and a lane count), so each sub-tile runs its far
 // sources on the offset path, 3 instead of 6 vector instructions per pair,
 // and the rest on the FP64-formed separation.  With ions in a spatial order
@@ -123,6 +126,18 @@ __device__ __forceinline__ void coulomb_fp64_exact_item(const CoulombArgs &a, in
 // The self pair has r = 0 and contributes exactly 0 through the eps term.
 // ---------------------------------------------------------------------------
 constexpr float kNearFactor = 0.25f;
+
+// LOC far test: the squared distance from the source offset (x, y, z) to the
+// box of half-extents (hx, hy, hz) around the origin, against
+// near2 = (kNearFactor h)^2.
+__device__ __forceinline__ bool far_from_box(float x, float y, float z, float hx, float hy,
+                                             float hz, float near2)
+{
+    const float ex = fmaxf(fabsf(x) - hx, 0.f);
+    const float ey = fmaxf(fabsf(y) - hy, 0.f);
+    const float ez = fmaxf(fabsf(z) - hz, 0.f);
+    return fmaf(ex, ex, fmaf(ey, ey, ez * ez)) >= near2;
+}
 
 __device__ __forceinline__ double wave_min(double v)
 {
@@ -195,7 +210,7 @@ __device__ __forceinline__ void coulomb_tiled_item(const CoulombArgs &a, int ite
     // thread), targets as FP32 offsets from it
     double ox = 0.0, oy = 0.0, oz = 0.0;
     float fx[R], fy[R], fz[R];
-    float far2 = 0.f;   // LOC: squared offset beyond which a source is far
+    float hx = 0.f, hy = 0.f, hz = 0.f, near2 = 0.f;   // LOC: target box and far test
     if (LOC) {
         double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
@@ -226,10 +241,10 @@ __device__ __forceinline__ void coulomb_tiled_item(const CoulombArgs &a, int ite
         ox = 0.5 * (mn[0] + mx[0]);
         oy = 0.5 * (mn[1] + mx[1]);
         oz = 0.5 * (mn[2] + mx[2]);
-        const float hx = (float)(0.5 * (mx[0] - mn[0]));
-        const float hy = (float)(0.5 * (mx[1] - mn[1]));
-        const float hz = (float)(0.5 * (mx[2] - mn[2]));
-        far2 = (1.f + kNearFactor) * (1.f + kNearFactor) * (hx * hx + hy * hy + hz * hz);
+        hx = (float)(0.5 * (mx[0] - mn[0]));
+        hy = (float)(0.5 * (mx[1] - mn[1]));
+        hz = (float)(0.5 * (mx[2] - mn[2]));
+        near2 = kNearFactor * kNearFactor * (hx * hx + hy * hy + hz * hz);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             fx[r] = (float)(px[r] - ox);
@@ -256,7 +271,7 @@ __device__ __forceinline__ void coulomb_tiled_item(const CoulombArgs &a, int ite
                 sz = a.sz[t0 + tid];
                 s4 = make_float4((float)(sx - ox), (float)(sy - oy), (float)(sz - oz),
                                  UNIT ? 1.f : a.sqf[t0 + tid]);
-                far_src = fmaf(s4.x, s4.x, fmaf(s4.y, s4.y, s4.z * s4.z)) >= far2;
+                far_src = far_from_box(s4.x, s4.y, s4.z, hx, hy, hz, near2);
             }
             const unsigned long long far_mask = __ballot(far_src);
             const int nfar = __popcll(far_mask);
@@ -432,20 +447,29 @@ __device__ __forceinline__ void coulomb_tiled_item(const CoulombArgs &a, int ite
 // near sources therefore no longer holds the others back at every tile, and
 // the global-load latency of the next sub-tile overlaps the sums.
 // ---------------------------------------------------------------------------
+// LDS of the streamed kernel: one region per work-group, whichever
+// instantiation of coulomb_stream_item runs the item (a non-template holder,
+// so the instantiations one kernel calls share it).
+struct StreamLds {
+    double x[kBlock], y[kBlock], z[kBlock];   // FP64 sources (near path); reused
+    float4 f[kBlock];                         // (offset or 0, weight)
+};
+__device__ __forceinline__ StreamLds &stream_lds()
+{
+    __shared__ StreamLds s;
+    return s;
+}
+
 template <int R, int MODE, bool UNIT, bool LOC>
-__device__ __forceinline__ void coulomb_stream_item(const CoulombArgs &a, int item)
+__device__ __forceinline__ void coulomb_stream_item(const CoulombArgs &a, const CoulombItem &it)
 {
     static_assert(MODE == KP_MIXED || MODE == KP_MIXED_KAHAN, "a mixed-precision kernel");
     constexpr int NW = kBlock / 64;      // waves per work-group: 4 source streams
     constexpr int W = 64;                // sources per sub-tile = lanes per wave
-    __shared__ double lx[NW * W], ly[NW * W], lz[NW * W];   // FP64 sources (near path); reused
-    __shared__ float4 lf[NW * W];                           // (offset or 0, weight)
+    StreamLds &lds = stream_lds();
+    double *const lx = lds.x, *const ly = lds.y, *const lz = lds.z;
+    float4 *const lf = lds.f;
 
-    const CoulombItem it = a.items[item];
-    // uniform over the work-group (one item), so either every wave reaches
-    // the closing barrier or none does
-    if (a.state && (!a.state[it.cloud].active || (a.skip_valid_k1 && a.state[it.cloud].k1_valid)))
-        return;
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     double *const wx = lx + wave * W;
@@ -474,7 +498,7 @@ __device__ __forceinline__ void coulomb_stream_item(const CoulombArgs &a, int it
     // LOC: origin = centre of the targets' bounding box, reduced inside the
     // wave (each wave holds every target, so all four find the same box)
     double ox = 0.0, oy = 0.0, oz = 0.0;
-    float far2 = 0.f;
+    float hx = 0.f, hy = 0.f, hz = 0.f, near2 = 0.f;
     if (LOC) {
         double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
@@ -493,10 +517,10 @@ __device__ __forceinline__ void coulomb_stream_item(const CoulombArgs &a, int it
         ox = 0.5 * (mn[0] + mx[0]);
         oy = 0.5 * (mn[1] + mx[1]);
         oz = 0.5 * (mn[2] + mx[2]);
-        const float hx = (float)(0.5 * (mx[0] - mn[0]));
-        const float hy = (float)(0.5 * (mx[1] - mn[1]));
-        const float hz = (float)(0.5 * (mx[2] - mn[2]));
-        far2 = (1.f + kNearFactor) * (1.f + kNearFactor) * (hx * hx + hy * hy + hz * hz);
+        hx = (float)(0.5 * (mx[0] - mn[0]));
+        hy = (float)(0.5 * (mx[1] - mn[1]));
+        hz = (float)(0.5 * (mx[2] - mn[2]));
+        near2 = kNearFactor * kNearFactor * (hx * hx + hy * hy + hz * hz);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             fx[r] = (float)(px[r] - ox);
@@ -542,7 +566,7 @@ __device__ __forceinline__ void coulomb_stream_item(const CoulombArgs &a, int it
             bool far_src = false;
             if (valid) {
                 s4 = make_float4((float)(sx - ox), (float)(sy - oy), (float)(sz - oz), UNIT ? 1.f : sq);
-                far_src = fmaf(s4.x, s4.x, fmaf(s4.y, s4.y, s4.z * s4.z)) >= far2;
+                far_src = far_from_box(s4.x, s4.y, s4.z, hx, hy, hz, near2);
             }
             const unsigned long long far_mask = __ballot(far_src);
             nfar = __popcll(far_mask);
@@ -666,6 +690,30 @@ __device__ __forceinline__ void coulomb_stream_item(const CoulombArgs &a, int it
             o[2 * a.stride] = k * Az[r];
         }
     }
+}
+
+
+// A streamed item with R target sets per thread: the last block of a cloud
+// may fill fewer than R sets of 64 targets, and the instantiation for the
+// sets it fills runs it, so the empty sets cost nothing (the sums of the
+// filled ones are the same, bit for bit).
+template <int R, int MODE, bool UNIT, bool LOC>
+__device__ __forceinline__ void coulomb_stream_sets(const CoulombArgs &a, int item)
+{
+    const CoulombItem it = a.items[item];
+    // uniform over the work-group (one item), so either every wave reaches
+    // the closing barrier or none does
+    if (a.state && (!a.state[it.cloud].active || (a.skip_valid_k1 && a.state[it.cloud].k1_valid)))
+        return;
+    const int sets = (it.i1 - it.i0 + 63) >> 6;
+    if constexpr (R >= 2) {
+        if (sets <= 1) return coulomb_stream_item<1, MODE, UNIT, LOC>(a, it);
+    }
+    if constexpr (R >= 4) {
+        if (sets <= 2) return coulomb_stream_item<2, MODE, UNIT, LOC>(a, it);
+        if (sets <= 3) return coulomb_stream_item<3, MODE, UNIT, LOC>(a, it);
+    }
+    coulomb_stream_item<R, MODE, UNIT, LOC>(a, it);
 }
 
 
@@ -1169,7 +1217,7 @@ __device__ __forceinline__ void persistent_item(const CoulombArgs &a, int k)
     else if constexpr (KIND == 1)
         coulomb_tiled_item<R, MODE, UNIT, 4, LOC>(a, k);
     else
-        coulomb_stream_item<R, MODE, UNIT, LOC>(a, k);
+        coulomb_stream_sets<R, MODE, UNIT, LOC>(a, k);
 }
 
 // Coulomb items of stage S, barrier, stage S, barrier; false once a barrier failed

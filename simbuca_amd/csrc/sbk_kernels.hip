@@ -19,7 +19,7 @@ __global__ __launch_bounds__(kBlock) void coulomb_tiled(CoulombArgs a)
 template <int R, int MODE, bool UNIT, bool LOC>
 __global__ __launch_bounds__(kBlock) void coulomb_stream(CoulombArgs a)
 {
-    coulomb_stream_item<R, MODE, UNIT, LOC>(a, xcd_item(blockIdx.x, gridDim.x));
+    coulomb_stream_sets<R, MODE, UNIT, LOC>(a, xcd_item(blockIdx.x, gridDim.x));
 }
 
 template <int S>
