@@ -36,11 +36,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOP_PER_PAIR = 20.0          # GPU Gems 3 n-body convention
-# HBM bytes per launch of the mixed Coulomb kernel at 65,536 ions with the table's shape
-# (4 targets/thread, 4 streams, 32 slices, local coordinates, streamed kernel), from
-# rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 in KiB per dispatch (the factors of a
-# 1 GiB device copy); the writes are the 32 slice partials, 32 x 3 x 8 B x 65,536
-MEASURED_TRAFFIC = {("gfx950", 65536, (4, 4, 32, 1, 1)): 2.0 * 8610.1 * 1024 + 1.0 * 49152.0 * 1024}
+# HBM bytes per launch of the mixed Coulomb kernel at 65,536 ions, per launch shape
+# (targets/thread, streams, slices, local coordinates, kernel), from rocprofv3
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 in KiB per dispatch (the factors of a 1 GiB
+# device copy); the writes are the 32 slice partials, 32 x 3 x 8 B x 65,536 (the
+# automatic split of this shape also makes 32 slices)
+MEASURED_TRAFFIC = {
+    ("gfx950", 65536, (4, 4, 32, 1, 1)): 2.0 * 8610.1 * 1024 + 1.0 * 49152.0 * 1024,
+    ("gfx950", 65536, (4, 4, 0, 1, 1)): 2.0 * 8640.6 * 1024 + 1.0 * 49152.0 * 1024,
+}
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector peak
 PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector peak
 HBM_PEAK_GBS = 8000.0
@@ -566,7 +570,10 @@ def main():
         "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32_TFLOPS,
         "traffic": MEASURED_TRAFFIC.get((info["name"].split(":")[0], n, tuple(tune or ()))),
-        "kernel": "coulomb_tiled<mixed>", "flop_per_pair": FLOP_PER_PAIR,
+        "kernel": ("coulomb_stream<mixed>" if tune and len(tune) > 4 and tune[4] == 1
+                   else "coulomb_tiled<mixed>" if tune and len(tune) > 4 and tune[4] == 0
+                   else "coulomb<mixed>"),
+        "flop_per_pair": FLOP_PER_PAIR,
         "pairs_per_launch": float(n) * (n - 1), "ms_per_launch": coul_ms,
         "coulomb_evals_timed": evals, "coulomb_launches_timed": st["coulomb_launches"],
         "fp64_kernel_frac_of_fp64_peak": FLOP_PER_PAIR * n * (n - 1)
