@@ -450,9 +450,12 @@ __device__ __forceinline__ void coulomb_tiled_item(const CoulombArgs &a, int ite
 // LDS of the streamed kernel: one region per work-group, whichever
 // instantiation of coulomb_stream_item runs the item (a non-template holder,
 // so the instantiations one kernel calls share it).
-struct StreamLds {
-    double x[kBlock], y[kBlock], z[kBlock];   // FP64 sources (near path); reused
-    float4 f[kBlock];                         // (offset or 0, weight)
+union StreamLds {
+    struct {
+        double x[kBlock], y[kBlock], z[kBlock];   // FP64 sources (near path); reused
+        float4 f[kBlock];                         // (offset or 0, weight)
+    } tile;
+    double sums[4][kBlock / 64][3][64];           // the streams' sums: [set][wave][component][lane]
 };
 __device__ __forceinline__ StreamLds &stream_lds()
 {
@@ -466,9 +469,10 @@ __device__ __forceinline__ void coulomb_stream_item(const CoulombArgs &a, const 
     static_assert(MODE == KP_MIXED || MODE == KP_MIXED_KAHAN, "a mixed-precision kernel");
     constexpr int NW = kBlock / 64;      // waves per work-group: 4 source streams
     constexpr int W = 64;                // sources per sub-tile = lanes per wave
+    static_assert(R <= 4, "one LDS region of sums per target set");
     StreamLds &lds = stream_lds();
-    double *const lx = lds.x, *const ly = lds.y, *const lz = lds.z;
-    float4 *const lf = lds.f;
+    double *const lx = lds.tile.x, *const ly = lds.tile.y, *const lz = lds.tile.z;
+    float4 *const lf = lds.tile.f;
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -662,33 +666,32 @@ __device__ __forceinline__ void coulomb_stream_item(const CoulombArgs &a, const 
             Az[r] = (double)Sz[r] - (double)Cz[r];
         }
     }
-    // the four streams' sums, wave 0 first, one target set at a time
+    // the four streams' sums of every target set in LDS at once (one barrier
+    // after the last sub-tile, one after the stores); wave r then adds set r's
+    // in the fixed order wave 0, 1, 2, 3 and writes it
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        __syncthreads();
-        lx[tid] = Ax[r];
-        ly[tid] = Ay[r];
-        lz[tid] = Az[r];
-        __syncthreads();
-        if (wave == 0) {
-            for (int q = 1; q < NW; ++q) {
-                Ax[r] += lx[q * W + lane];
-                Ay[r] += ly[q * W + lane];
-                Az[r] += lz[q * W + lane];
-            }
-        }
+        lds.sums[r][wave][0][lane] = Ax[r];
+        lds.sums[r][wave][1][lane] = Ay[r];
+        lds.sums[r][wave][2][lane] = Az[r];
     }
-    if (wave != 0) return;
+    __syncthreads();
+    if (wave >= R) return;
+    const int g = it.i0 + wave * W + lane;
+    if (g < it.i1) {
+        double s[3];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int g = it.i0 + r * W + lane;
-        if (g < it.i1) {
-            const double k = a.kc ? a.kc[g] : 1.0;
-            double *o = a.out + (long long)it.slice * 3 * a.stride + g;
-            o[0] = k * Ax[r];
-            o[a.stride] = k * Ay[r];
-            o[2 * a.stride] = k * Az[r];
+        for (int c = 0; c < 3; ++c) {
+            s[c] = lds.sums[wave][0][c][lane];
+#pragma unroll
+            for (int q = 1; q < NW; ++q) s[c] += lds.sums[wave][q][c][lane];
         }
+        const double k = a.kc ? a.kc[g] : 1.0;
+        double *o = a.out + (long long)it.slice * 3 * a.stride + g;
+        o[0] = k * s[0];
+        o[a.stride] = k * s[1];
+        o[2 * a.stride] = k * s[2];
     }
 }
 
