@@ -434,8 +434,11 @@ def test_stream_kernel_equals_tiled_kernel(rng, precision, unit):
     order, folds them in the same order and combines the four waves in the
     same order: its sums are the tiled kernel's bit for bit, for every shape
     it supports, and within 1e-6 of the FP64-exact sum.  Clouds of uneven
-    sizes (ragged last tiles, a 3-ion cloud) share the launch."""
-    sizes = [3001, 64, 3, 700]
+    sizes (ragged last tiles, a 3-ion cloud) share the launch; with 4 targets
+    per thread their last blocks fill 3 (3001, 700), 1 (64, 3) and 2 (356)
+    sets of 64 targets, which the streamed kernel runs with the instantiation
+    for those sets."""
+    sizes = [3001, 64, 3, 700, 356]
     sf, _ = _both_fields(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0)
     clouds = []
     for n in sizes:
