@@ -61,17 +61,21 @@ Plan plan_coulomb(const std::vector<PlanCloud> &clouds, int blk, int quantum, in
     }
     // Work-group b runs on XCD b % 8 and takes the b / 8-th item of that
     // XCD's contiguous run (xcd_item in the kernels); an XCD starts its
-    // work-groups in order.  Each run is therefore ordered costliest first
-    // (an item costs its source count: every lane loops over all of them),
+    // work-groups in order.  Each run is therefore ordered costliest first,
     // so that short items fill the last, partly empty round of the XCD; the
     // runs themselves keep their clouds, which keeps each XCD's sources in
-    // its L2.
+    // its L2.  An item costs its sources times the sets of 64 targets it
+    // fills: every lane loops over all the sources, and the streamed kernel
+    // runs a cloud's last block with only the sets it fills.
     const int nb = (int)L.items.size(), qx = nb / 8, rx = nb % 8;
+    auto cost = [](const CoulombItem &u) {
+        return (long long)((u.i1 - u.i0 + 63) / 64) * (u.j1 - u.j0);
+    };
     for (int x = 0; x < 8; ++x) {
         const int b0 = x * qx + std::min(x, rx), cnt = qx + (x < rx ? 1 : 0);
         std::stable_sort(L.items.begin() + b0, L.items.begin() + b0 + cnt,
-                         [](const CoulombItem &u, const CoulombItem &v) {
-                             return u.j1 - u.j0 > v.j1 - v.j0;
+                         [&cost](const CoulombItem &u, const CoulombItem &v) {
+                             return cost(u) > cost(v);
                          });
     }
     return L;
