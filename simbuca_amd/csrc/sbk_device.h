@@ -29,7 +29,9 @@ constexpr double kPi = 3.14159265358979323;
 // ---------------------------------------------------------------------------
 // Work-group -> work-item mapping.  Work-groups are dealt round-robin to the
 // 8 XCDs; this gives each XCD one contiguous run of items, so the items of
-// one source slice (which reuse the same sources) share an L2.
+// one cloud and source slice (which reuse the same sources) share an L2.
+// The engine's lists hold 8 equally long runs cut at equal work, the
+// shorter ones padded with empty items (i0 == i1) that return at once.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int xcd_item(int b, int nb)
 {
@@ -48,6 +50,9 @@ __device__ __forceinline__ void coulomb_fp64_exact_item(const CoulombArgs &a, in
 {
     __shared__ double lx[kBlock], ly[kBlock], lz[kBlock], lq[kBlock];
     const CoulombItem it = a.items[item];
+    // an empty item pads an XCD's run of the list; the checks are uniform
+    // over the work-group, so either every wave reaches the barriers or none
+    if (it.i0 >= it.i1) return;
     if (a.state && (!a.state[it.cloud].active || (a.skip_valid_k1 && a.state[it.cloud].k1_valid)))
         return;
     const int tid = threadIdx.x;
@@ -184,6 +189,9 @@ __device__ __forceinline__ void coulomb_tiled_item(const CoulombArgs &a, int ite
     __shared__ int lfar[LOC ? kBlock / kSubTile : 1];       // LOC: far sources per sub-tile
 
     const CoulombItem it = a.items[item];
+    // an empty item pads an XCD's run of the list; the checks are uniform
+    // over the work-group, so either every wave reaches the barriers or none
+    if (it.i0 >= it.i1) return;
     if (a.state && (!a.state[it.cloud].active || (a.skip_valid_k1 && a.state[it.cloud].k1_valid)))
         return;
     const int tid = threadIdx.x;
@@ -705,7 +713,8 @@ __device__ __forceinline__ void coulomb_stream_sets(const CoulombArgs &a, int it
 {
     const CoulombItem it = a.items[item];
     // uniform over the work-group (one item), so either every wave reaches
-    // the closing barrier or none does
+    // the closing barrier or none does; an empty item pads an XCD's run
+    if (it.i0 >= it.i1) return;
     if (a.state && (!a.state[it.cloud].active || (a.skip_valid_k1 && a.state[it.cloud].k1_valid)))
         return;
     const int sets = (it.i1 - it.i0 + 63) >> 6;

@@ -61,23 +61,47 @@ Plan plan_coulomb(const std::vector<PlanCloud> &clouds, int blk, int quantum, in
     }
     // Work-group b runs on XCD b % 8 and takes the b / 8-th item of that
     // XCD's contiguous run (xcd_item in the kernels); an XCD starts its
-    // work-groups in order.  Each run is therefore ordered costliest first,
-    // so that short items fill the last, partly empty round of the XCD; the
-    // runs themselves keep their clouds, which keeps each XCD's sources in
-    // its L2.  An item costs its sources times the sets of 64 targets it
-    // fills: every lane loops over all the sources, and the streamed kernel
-    // runs a cloud's last block with only the sets it fills.
-    const int nb = (int)L.items.size(), qx = nb / 8, rx = nb % 8;
+    // work-groups in order.  The list is cut into eight contiguous runs of
+    // about equal work, which keeps each cloud's items, and so its sources,
+    // in one or two XCDs' L2; each run is ordered costliest first, so that
+    // short items fill the XCD's last, partly empty round.  An item costs its
+    // sources times the sets of 64 targets it fills: every lane loops over
+    // all the sources, and the streamed kernel runs a cloud's last block with
+    // only the sets it fills.  Runs of unequal counts are fine: the grid is
+    // 8 times the longest, and the work-groups past a run's end leave.
+    const int nb = (int)L.items.size();
     auto cost = [](const CoulombItem &u) {
         return (long long)((u.i1 - u.i0 + 63) / 64) * (u.j1 - u.j0);
     };
+    long long total = 0;
+    for (const CoulombItem &u : L.items) total += cost(u);
+    L.xrun.assign(9, nb);
+    L.xrun[0] = 0;
+    long long acc = 0;
+    for (int k = 0, x = 1; k < nb && x < 8; ++k) {
+        acc += cost(L.items[k]);
+        while (x < 8 && acc * 8 >= total * x) L.xrun[x++] = k + 1;
+    }
+    int longest = 0;
     for (int x = 0; x < 8; ++x) {
-        const int b0 = x * qx + std::min(x, rx), cnt = qx + (x < rx ? 1 : 0);
-        std::stable_sort(L.items.begin() + b0, L.items.begin() + b0 + cnt,
+        std::stable_sort(L.items.begin() + L.xrun[x], L.items.begin() + L.xrun[x + 1],
                          [&cost](const CoulombItem &u, const CoulombItem &v) {
                              return cost(u) > cost(v);
                          });
+        longest = std::max(longest, L.xrun[x + 1] - L.xrun[x]);
     }
+    // Lay the runs out at x * longest and pad each to `longest` with empty
+    // items, so that the kernels' equal-count mapping (8 runs of `longest`)
+    // gives work-group b the (b / 8)-th item of XCD b % 8's run.
+    if (nb > 0) {
+        std::vector<CoulombItem> laid((size_t)8 * longest);   // value-initialized: empty
+        for (int x = 0; x < 8; ++x)
+            std::copy(L.items.begin() + L.xrun[x], L.items.begin() + L.xrun[x + 1],
+                      laid.begin() + (size_t)x * longest);
+        for (int x = 0; x < 9; ++x) L.xrun[x] = std::min(x, 8) * longest;
+        L.items.swap(laid);
+    }
+    L.grid = 8 * longest;
     return L;
 }
 
@@ -85,6 +109,11 @@ std::string check_plan(const std::vector<PlanCloud> &clouds, const Plan &p)
 {
     char buf[256];
     if (p.slices.size() != clouds.size()) return "one slice count per cloud is needed";
+    // the XCD runs: 8 equally long runs, x at x * grid / 8, that make up the list
+    const int nb = (int)p.items.size();
+    if (p.grid != nb || nb % 8 != 0 || p.xrun.size() != 9) return "the list is not 8 equal XCD runs";
+    for (int x = 0; x <= 8; ++x)
+        if (p.xrun[x] != x * (nb / 8)) return "the XCD runs do not start at multiples of grid / 8";
     for (size_t ci = 0; ci < clouds.size(); ++ci)
         if (p.slices[ci] < 1 || p.slices[ci] > p.max_slices) {
             std::snprintf(buf, sizeof(buf), "cloud %zu has %d slices, max_slices %d", ci,
@@ -95,6 +124,8 @@ std::string check_plan(const std::vector<PlanCloud> &clouds, const Plan &p)
     std::vector<long long> covered(clouds.size(), 0);
     for (size_t k = 0; k < p.items.size(); ++k) {
         const CoulombItem &w = p.items[k];
+        if (w.i0 == w.i1 && w.i0 == 0 && w.j0 == 0 && w.j1 == 0 && w.cloud == 0 && w.slice == 0)
+            continue;   // an empty item padding an XCD's run
         if (w.cloud < 0 || w.cloud >= (int)clouds.size()) return "item names no cloud";
         const PlanCloud &c = clouds[w.cloud];
         const int lo = c.off, hi = c.off + c.n;

@@ -25,6 +25,8 @@ struct Plan {
     std::vector<CoulombItem> items;
     std::vector<int> slices;   // per cloud (1 for clouds without items)
     int max_slices = 1;
+    std::vector<int> xrun;     // 9 entries: XCD x takes items [xrun[x], xrun[x+1]) = x * grid / 8 ...
+    int grid = 0;              // work-groups to launch = items, 8 runs padded with empty items
 };
 
 // targets_per_block: targets of one work-group (kBlock / groups * targets per
@@ -37,7 +39,8 @@ Plan plan_coulomb(const std::vector<PlanCloud> &clouds, int targets_per_block, i
 
 // Every item targets and sources one cloud inside its range, its slice is
 // below the cloud's slice count and that count at most max_slices; each
-// cloud's items cover every (target, source) pair exactly once.  Returns
+// cloud's items cover every (target, source) pair exactly once; the list is
+// 8 equally long XCD runs, in which empty items (all zero) only pad.  Returns
 // an empty string when all hold, else what failed.
 std::string check_plan(const std::vector<PlanCloud> &clouds, const Plan &p);
 

@@ -14,6 +14,7 @@
 //   host_check <scratch dir> [<ini file>...]
 // Prints "host_check ok" and exits 0 when every check passes; a sanitizer
 // report ends the process with an error first.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -316,6 +317,23 @@ static void check_planner()
                             if (!bad.empty()) std::fprintf(stderr, "plan: %s\n", bad.c_str());
                             CHECK(bad.empty());
                             if (exact) CHECK(p.max_slices == 1);
+                            // the XCD runs hold about equal work: none exceeds an eighth of
+                            // the list's cost by more than its costliest item
+                            {
+                                auto cost = [](const sbk::CoulombItem &u) {
+                                    return (long long)((u.i1 - u.i0 + 63) / 64) * (u.j1 - u.j0);
+                                };
+                                long long total = 0, top = 0;
+                                for (const auto &u : p.items) {
+                                    total += cost(u);
+                                    top = std::max(top, cost(u));
+                                }
+                                for (int x = 0; x < 8 && !p.items.empty(); ++x) {
+                                    long long run = 0;
+                                    for (int k = p.xrun[x]; k < p.xrun[x + 1]; ++k) run += cost(p.items[k]);
+                                    CHECK(8 * run <= total + 8 * top);
+                                }
+                            }
                             lists++;
                             if (!p.items.empty() && lists % 7 == 0) {
                                 sbk::Plan q1 = p;
@@ -324,6 +342,9 @@ static void check_planner()
                                 sbk::Plan q2 = p;
                                 q2.items.pop_back();
                                 CHECK(!sbk::check_plan(pc, q2).empty());
+                                sbk::Plan q3 = p;   // a grid that misses the longest run's last item
+                                q3.grid -= 8;
+                                CHECK(!sbk::check_plan(pc, q3).empty());
                             }
                         }
     }
