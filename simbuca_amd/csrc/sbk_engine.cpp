@@ -398,7 +398,10 @@ int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
         pc[ci].off = e->clouds[ci].off;
         pc[ci].on = !only_coulomb || e->clouds[ci].f.coulomb;
     }
-    sbk::Plan plan = sbk::plan_coulomb(pc, blk, sbk::kSubTile * q, sreq, capacity, exact);
+    // the streamed kernel runs a block's filled sets of 64 targets only, so
+    // mid-size clouds may take half-size blocks (sbk_plan.h)
+    const int split_blk = (kern == sbk::KK_STREAM && tpt == 4) ? blk / 2 : 0;
+    sbk::Plan plan = sbk::plan_coulomb(pc, blk, sbk::kSubTile * q, sreq, capacity, exact, split_blk);
     const std::string bad = sbk::check_plan(pc, plan);
     if (!bad.empty()) return fail(SBK_ESTATE, "Coulomb work list is inconsistent: %s", bad.c_str());
     ItemList L;

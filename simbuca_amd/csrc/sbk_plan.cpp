@@ -7,21 +7,17 @@
 namespace sbk {
 
 Plan plan_coulomb(const std::vector<PlanCloud> &clouds, int blk, int quantum, int sreq,
-                  long long capacity, bool exact)
+                  long long capacity, bool exact, int split_blk)
 {
     Plan L;
     L.slices.assign(clouds.size(), 1);
-    long long total_blocks = 0;
-    for (const PlanCloud &c : clouds)
-        if (c.n > 0 && c.on) total_blocks += (c.n + blk - 1) / blk;
     // automatic split: about eight rounds of resident work-groups; short
     // items keep the last, partially filled round (the tail) short
     const long long target = 8 * capacity;
-    for (size_t ci = 0; ci < clouds.size(); ++ci) {
-        const PlanCloud &c = clouds[ci];
-        if (c.n <= 0 || !c.on) continue;
-        const int ib = (c.n + blk - 1) / blk;
-        int S = 1;
+    // slice count and length of a cloud of n ions when the list holds
+    // total_blocks target blocks
+    auto split = [&](int n, long long total_blocks, int &S, int &chunk) {
+        S = 1;
         if (!exact) {
             if (sreq > 0)
                 S = sreq;
@@ -29,17 +25,54 @@ Plan plan_coulomb(const std::vector<PlanCloud> &clouds, int blk, int quantum, in
                 long long want = total_blocks > 0 ? (target + total_blocks / 2) / total_blocks : 1;
                 // at least 512 sources per slice (partial sums stay a few % of the traffic),
                 // relaxed to 64 for clouds too small to fill the device otherwise
-                long long cap = std::max(1, c.n / (total_blocks * 8 < capacity ? 64 : 512));
+                long long cap = std::max(1, n / (total_blocks * 8 < capacity ? 64 : 512));
                 S = (int)std::max(1LL, std::min(want, cap));
             }
         }
-        S = std::max(1, std::min(S, c.n));
+        S = std::max(1, std::min(S, n));
         // slice length: a multiple of the sources one LDS tile gives all wave
         // groups (each group takes its own kSubTile of every tile), so that
         // only a cloud's last slice ends in a partial tile
-        int chunk = (c.n + S - 1) / S;
+        chunk = (n + S - 1) / S;
         if (!exact) chunk = ((chunk + quantum - 1) / quantum) * quantum;
-        S = (c.n + chunk - 1) / chunk;
+        S = (n + chunk - 1) / chunk;
+    };
+    // Target blocks per cloud.  The slice cap keeps the items of mid-size
+    // clouds large (a few hundred sources against a whole block); when the
+    // kernel runs a block's filled target sets only (split_blk > 0), a cloud
+    // whose items would each cost more than a fifth of one resident
+    // work-group's share of the evaluation takes blocks of split_blk
+    // targets instead, which gives the last rounds smaller items without
+    // shortening any source stream.  A single large cloud never qualifies:
+    // its items are an eighth of the share.
+    std::vector<int> cblk(clouds.size(), blk);
+    long long total_blocks = 0, pairs = 0;
+    for (const PlanCloud &c : clouds)
+        if (c.n > 0 && c.on) {
+            total_blocks += (c.n + blk - 1) / blk;
+            pairs += (long long)c.n * c.n;
+        }
+    if (!exact && sreq == 0 && split_blk > 0 && split_blk < blk && capacity > 0) {
+        for (size_t ci = 0; ci < clouds.size(); ++ci) {
+            const PlanCloud &c = clouds[ci];
+            if (c.n <= 0 || !c.on) continue;
+            int S, chunk;
+            split(c.n, total_blocks, S, chunk);
+            const long long item = (long long)std::min(blk, c.n) * std::min(chunk, c.n);
+            if (5 * item * capacity > pairs) cblk[ci] = split_blk;
+        }
+        total_blocks = 0;
+        for (size_t ci = 0; ci < clouds.size(); ++ci)
+            if (clouds[ci].n > 0 && clouds[ci].on)
+                total_blocks += (clouds[ci].n + cblk[ci] - 1) / cblk[ci];
+    }
+    for (size_t ci = 0; ci < clouds.size(); ++ci) {
+        const PlanCloud &c = clouds[ci];
+        if (c.n <= 0 || !c.on) continue;
+        const int cb = cblk[ci];
+        const int ib = (c.n + cb - 1) / cb;
+        int S, chunk;
+        split(c.n, total_blocks, S, chunk);
         // slice boundaries (relative to the cloud's first ion)
         std::vector<int> cut;
         for (int s = 0; s <= S; ++s) cut.push_back(std::min(c.n, s * chunk));
@@ -50,8 +83,8 @@ Plan plan_coulomb(const std::vector<PlanCloud> &clouds, int blk, int quantum, in
             for (int b = 0; b < ib; ++b) {
                 CoulombItem w{};
                 w.cloud = (int)ci;
-                w.i0 = c.off + b * blk;
-                w.i1 = c.off + std::min(c.n, (b + 1) * blk);
+                w.i0 = c.off + b * cb;
+                w.i1 = c.off + std::min(c.n, (b + 1) * cb);
                 w.j0 = j0;
                 w.j1 = j1;
                 w.slice = s;

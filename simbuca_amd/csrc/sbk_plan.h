@@ -34,8 +34,11 @@ struct Plan {
 // that the items make about eight rounds of `capacity` resident work-groups.
 // quantum: a slice's length is a multiple of it (kSubTile * groups).
 // exact: one slice per cloud (the FP64-exact sum is serial over sources).
+// split_blk > 0 (only for kernels that run a block's filled target sets):
+// with the automatic split, clouds whose items would be large against one
+// work-group's share take blocks of split_blk targets.
 Plan plan_coulomb(const std::vector<PlanCloud> &clouds, int targets_per_block, int quantum,
-                  int slices_req, long long capacity, bool exact);
+                  int slices_req, long long capacity, bool exact, int split_blk = 0);
 
 // Every item targets and sources one cloud inside its range, its slice is
 // below the cloud's slice count and that count at most max_slices; each

@@ -309,10 +309,12 @@ static void check_planner()
             for (int tpt : {1, 2, 4, 8})
                 for (int sreq : {0, 1, 3, 7, 64, 100000})
                     for (long long cap : {256LL, 1024LL, 4096LL})
-                        for (int exact : {0, 1}) {
+                        for (int exact : {0, 1})
+                        for (int split : {0, 1}) {
                             const int blk = sbk::kBlock / q * tpt;
                             sbk::Plan p = sbk::plan_coulomb(pc, exact ? sbk::kBlock : blk,
-                                                            sbk::kSubTile * q, sreq, cap, exact != 0);
+                                                            sbk::kSubTile * q, sreq, cap, exact != 0,
+                                                            split ? blk / 2 : 0);
                             const std::string bad = sbk::check_plan(pc, p);
                             if (!bad.empty()) std::fprintf(stderr, "plan: %s\n", bad.c_str());
                             CHECK(bad.empty());
