@@ -73,6 +73,15 @@ Plan plan_coulomb(const std::vector<PlanCloud> &clouds, int blk, int quantum, in
         const int ib = (c.n + cb - 1) / cb;
         int S, chunk;
         split(c.n, total_blocks, S, chunk);
+        // the item-cost cap (sbk_plan.h): below kMinCapChunk sources a
+        // slice's per-item set-up costs more than the tail it saves
+        if (!exact && capacity > 0 && pairs > 0) {
+            const long long tb = std::min(cb, c.n);
+            while (chunk >= 2 * kMinCapChunk && kItemShare * tb * chunk * capacity > pairs) {
+                chunk = ((chunk / 2 + quantum - 1) / quantum) * quantum;
+                S = (c.n + chunk - 1) / chunk;
+            }
+        }
         // slice boundaries (relative to the cloud's first ion)
         std::vector<int> cut;
         for (int s = 0; s <= S; ++s) cut.push_back(std::min(c.n, s * chunk));

@@ -29,6 +29,15 @@ struct Plan {
     int grid = 0;              // work-groups to launch = items, 8 runs padded with empty items
 };
 
+// Item-cost cap: unless exact, a cloud whose items would each cost more than
+// 1/kItemShare of one resident work-group's share of the evaluation (its
+// target block times its slice, against sum n^2 / capacity) gets its slices
+// halved while each keeps at least kMinCapChunk sources (16 sub-tiles: 4 for
+// each of 4 source streams).  Clouds of unequal sizes give items of unequal
+// cost, and the longest set the tail of the dispatch.
+constexpr long long kItemShare = 4;
+constexpr int kMinCapChunk = 16 * kSubTile;
+
 // targets_per_block: targets of one work-group (kBlock / groups * targets per
 // thread).  slices_req > 0 fixes the slice count; 0 splits automatically so
 // that the items make about eight rounds of `capacity` resident work-groups.

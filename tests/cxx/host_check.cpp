@@ -319,6 +319,20 @@ static void check_planner()
                             if (!bad.empty()) std::fprintf(stderr, "plan: %s\n", bad.c_str());
                             CHECK(bad.empty());
                             if (exact) CHECK(p.max_slices == 1);
+                            // the item-cost cap: a cloud's first item (a whole first block
+                            // against its first, longest slice) is at most 1/kItemShare of a
+                            // resident work-group's share, or its slice cannot be halved
+                            if (!exact) {
+                                long long pairs = 0;
+                                for (const auto &c : pc)
+                                    if (c.on && c.n > 0) pairs += (long long)c.n * c.n;
+                                for (const auto &u : p.items) {
+                                    if (u.i0 == u.i1 || u.slice != 0 || u.i0 != pc[u.cloud].off) continue;
+                                    const long long tb = u.i1 - u.i0, chunk = u.j1 - u.j0;
+                                    CHECK(chunk < 2 * sbk::kMinCapChunk ||
+                                          sbk::kItemShare * tb * chunk * cap <= pairs);
+                                }
+                            }
                             // the XCD runs hold about equal work: none exceeds an eighth of
                             // the list's cost by more than its costliest item
                             {
