@@ -309,15 +309,16 @@ int  sbk_eval_force(sbk_engine *e, int cloud, double *derivs);
 int  sbk_eval_coulomb(sbk_engine *e, int cloud, int precision, double *out);
 
 /* Coulomb launch shape.  targets_per_thread in {1,2,4,8}, or 0 (the default:
- * 4 when the pair-weighted mean cloud size is >= 16384 ions with local
- * coordinates or >= 32768 without, else 2 when the clouds hold >= 2.5e8
- * pairs in total, else 1); source_groups in
+ * 4 when the pair-weighted mean cloud size is >= 32768 ions, else 2 when the
+ * clouds hold >= 2.5e8 pairs in total or local coordinates are on, else 1);
+ * source_groups in
  * {1,4}: wave groups of a 256-thread work-group that share one set of
  * targets and split each source tile; slices = number of source slices per
  * target block (0 = automatic); local_coords (spatial_order mixed engines):
  * 1 lets far sources form separations from FP32 offsets, 0 keeps
  * FP64-formed separations everywhere, -1 (automatic) chooses 1 when the
- * clouds' pair-weighted mean size is at least 16384 ions.  Applies to
+ * clouds hold >= 8e7 pairs in total and their pair-weighted mean size is at
+ * least 2048 ions.  Applies to
  * SBK_FP64, SBK_FP64_RSQ and SBK_MIXED*; SBK_FP64_EXACT always uses one thread per target
  * and one slice. */
 int  sbk_set_launch(sbk_engine *e, int targets_per_thread, int source_groups, int slices,

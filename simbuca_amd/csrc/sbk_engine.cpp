@@ -322,11 +322,15 @@ void free_lists(sbk_engine *e)
 }
 
 // Whether the mixed Coulomb kernel uses FP32 offsets for far sources.  It
-// pays off once a target block is a small part of its cloud (measured on
-// MI355X with the automatic shape: 13% faster at 20,000 ions, 22-23% at
-// 32,768-65,536, and 23-45% slower at 10,000 and below, where most sources
-// are near and the per-work-group set-up dominates), hence the automatic
-// rule on the pair-weighted mean cloud size.
+// pays off once a target block is a small part of its cloud and the work
+// items are long.  Fitted to autotuner runs on MI355X (each shape timed with
+// the read-back of its partial sums, automatic slices): on for single clouds
+// of 10,000-14,000 ions (5-16% faster than off) and ensembles whose
+// pair-weighted mean cloud size is 2,800-12,000 ions (13-30%); off for single
+// clouds of 500-8,000 ions (off 2-24% faster: most sources are near a small
+// cloud's blocks, and the items, split finely to fill the device, are short)
+// and for ensembles of 64-512-ion clouds (off 19% faster).  Hence both the
+// total pairs and the pair-weighted mean cloud size must be large.
 int local_coords(const sbk_engine *e, int precision)
 {
     if (!e->spatial || (precision != SBK_MIXED && precision != SBK_MIXED_KAHAN)) return 0;
@@ -336,15 +340,16 @@ int local_coords(const sbk_engine *e, int precision)
         n2 += (double)c.n * c.n;
         n3 += (double)c.n * c.n * c.n;
     }
-    return n2 > 0.0 && n3 / n2 >= 16384.0 ? 1 : 0;
+    return n2 >= 8e7 && n3 / n2 >= 2048.0 ? 1 : 0;
 }
 
 // Targets per thread.  0 (the default) picks by size, fitted to MI355X
-// measurements: 4 once a cloud fills the device with long work items (pair-
-// weighted mean size >= 16384 with local coordinates, whose far path reuses
-// each staged source over 4 targets best: 16,384-32,768 ions 2-6% faster
-// than 2; >= 32768 without), 2 for ensembles with many pairs in total, 1
-// for small clouds, where more, shorter work-groups win.
+// autotuner runs: 4 once a cloud fills the device with long work items
+// (pair-weighted mean size >= 32,768: 65,536 ions 3% faster than 2, 32,768
+// ions 1%; at 20,000 ions 2 is 5% faster than 4); 2 with local coordinates
+// (single clouds of 10,000-20,000 ions and ensembles of 2,800-12,000-ion
+// clouds: the far path's box of 128 targets) and for many pairs in total;
+// 1 for small clouds, where more, shorter work-groups win.
 int targets_per_thread(const sbk_engine *e, int precision)
 {
     if (e->tpt > 0) return e->tpt;
@@ -353,9 +358,8 @@ int targets_per_thread(const sbk_engine *e, int precision)
         n2 += (double)c.n * c.n;
         n3 += (double)c.n * c.n * c.n;
     }
-    const double big = local_coords(e, precision) ? 16384.0 : 32768.0;
-    if (n2 > 0.0 && n3 / n2 >= big) return 4;
-    return n2 >= 2.5e8 ? 2 : 1;
+    if (n2 > 0.0 && n3 / n2 >= 32768.0) return 4;
+    return (n2 >= 2.5e8 || local_coords(e, precision)) ? 2 : 1;
 }
 
 // Coulomb kernel of a launch: the streamed kernel exists for the mixed

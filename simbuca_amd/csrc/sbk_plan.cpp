@@ -14,6 +14,15 @@ Plan plan_coulomb(const std::vector<PlanCloud> &clouds, int blk, int quantum, in
     // automatic split: about eight rounds of resident work-groups; short
     // items keep the last, partially filled round (the tail) short
     const long long target = 8 * capacity;
+    // target blocks, pairs, and the items the list would hold with slices of
+    // 512 sources
+    long long total_blocks = 0, pairs = 0, items512 = 0;
+    for (const PlanCloud &c : clouds)
+        if (c.n > 0 && c.on) {
+            total_blocks += (c.n + blk - 1) / blk;
+            pairs += (long long)c.n * c.n;
+            items512 += (long long)((c.n + blk - 1) / blk) * std::max(1, c.n / 512);
+        }
     // slice count and length of a cloud of n ions when the list holds
     // total_blocks target blocks
     auto split = [&](int n, long long total_blocks, int &S, int &chunk) {
@@ -24,8 +33,9 @@ Plan plan_coulomb(const std::vector<PlanCloud> &clouds, int blk, int quantum, in
             else {
                 long long want = total_blocks > 0 ? (target + total_blocks / 2) / total_blocks : 1;
                 // at least 512 sources per slice (partial sums stay a few % of the traffic),
-                // relaxed to 64 for clouds too small to fill the device otherwise
-                long long cap = std::max(1, n / (total_blocks * 8 < capacity ? 64 : 512));
+                // relaxed to 64 when items of 512 sources would not fill one round of
+                // the device (single clouds up to about 5,000 ions)
+                long long cap = std::max(1, n / (items512 < capacity ? 64 : 512));
                 S = (int)std::max(1LL, std::min(want, cap));
             }
         }
@@ -46,12 +56,6 @@ Plan plan_coulomb(const std::vector<PlanCloud> &clouds, int blk, int quantum, in
     // shortening any source stream.  A single large cloud never qualifies:
     // its items are an eighth of the share.
     std::vector<int> cblk(clouds.size(), blk);
-    long long total_blocks = 0, pairs = 0;
-    for (const PlanCloud &c : clouds)
-        if (c.n > 0 && c.on) {
-            total_blocks += (c.n + blk - 1) / blk;
-            pairs += (long long)c.n * c.n;
-        }
     if (!exact && sreq == 0 && split_blk > 0 && split_blk < blk && capacity > 0) {
         for (size_t ci = 0; ci < clouds.size(); ++ci) {
             const PlanCloud &c = clouds[ci];
