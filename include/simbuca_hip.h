@@ -331,6 +331,14 @@ int  sbk_set_launch(sbk_engine *e, int targets_per_thread, int source_groups, in
  * to rounding; only the speed differs.  Other launches use the tiled kernel. */
 enum { SBK_KERNEL_AUTO = -1, SBK_KERNEL_TILED = 0, SBK_KERNEL_STREAM = 1 };
 int  sbk_set_coulomb_kernel(sbk_engine *e, int kernel);
+/* The launch the engine runs for `precision` with its current clouds and
+ * settings, the automatic choices resolved: targets per thread, source
+ * groups, the largest slice count of any cloud in the work list, local
+ * coordinates (0/1) and the Coulomb kernel (SBK_KERNEL_TILED or
+ * SBK_KERNEL_STREAM).  Any output pointer may be null.  No counterpart in
+ * the reference, whose launch parameters were fixed by its .dat tables. */
+int  sbk_get_launch(sbk_engine *e, int precision, int *targets_per_thread, int *source_groups,
+                    int *max_slices, int *local_coords, int *kernel);
 /* Times `reps` launches of the Coulomb kernel of the current state's
  * positions for every cloud; ms_per_launch receives the mean. */
 int  sbk_time_coulomb(sbk_engine *e, int precision, int reps, double *ms_per_launch);

@@ -129,7 +129,7 @@ EXPORTS = [
     "sbk_default_config", "sbk_default_fields", "sbk_create", "sbk_destroy", "sbk_add_cloud",
     "sbk_set_fields", "sbk_set_step", "sbk_advance", "sbk_advance_until", "sbk_step", "sbk_attempts",
     "sbk_synchronize", "sbk_reorder", "sbk_set_crossing_plane", "sbk_take_crossings", "sbk_get_state", "sbk_get_status", "sbk_get_stats", "sbk_reset_stats",
-    "sbk_eval_force", "sbk_eval_coulomb", "sbk_set_launch", "sbk_set_coulomb_kernel",
+    "sbk_eval_force", "sbk_eval_coulomb", "sbk_set_launch", "sbk_set_coulomb_kernel", "sbk_get_launch",
     "sbk_time_coulomb", "sbk_autotune",
     "sbk_load_tuning", "sbk_spatial_order", "sbk_writer_open", "sbk_writer_particles",
     "sbk_writer_cloud", "sbk_writer_set_ion", "sbk_writer_x", "sbk_writer_close", "sbk_import", "sbk_init_cloud", "sbk_mass_table_open",
@@ -185,6 +185,7 @@ def lib():
         "sbk_eval_coulomb": (c_int, [vp, c_int, c_int, dp]),
         "sbk_set_launch": (c_int, [vp, c_int, c_int, c_int, c_int]),
         "sbk_set_coulomb_kernel": (c_int, [vp, c_int]),
+        "sbk_get_launch": (c_int, [vp, c_int, ip, ip, ip, ip, ip]),
         "sbk_time_coulomb": (c_int, [vp, c_int, c_int, dp]),
         "sbk_autotune": (c_int, [vp, c_int, ip, ip, ip, ip, ip, ctypes.c_char_p]),
         "sbk_load_tuning": (c_int, [vp, c_int, ctypes.c_char_p]),
@@ -643,6 +644,14 @@ class Engine:
                                   ctypes.byref(lc), ctypes.byref(k),
                                   table_path.encode() if table_path else None))
         return t.value, q.value, s.value, lc.value, k.value
+
+    def launch(self, precision="mixed"):
+        """The launch the engine runs for `precision`, automatic choices resolved:
+        (targets/thread, source groups, largest slice count, local coords, kernel)."""
+        p = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        v = [ctypes.c_int() for _ in range(5)]
+        _check(lib().sbk_get_launch(self._h, p, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def set_shape(self, shape):
         """Applies (targets/thread, source groups, slices, local coords[, kernel])."""

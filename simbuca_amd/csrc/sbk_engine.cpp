@@ -1570,6 +1570,25 @@ int sbk_set_coulomb_kernel(sbk_engine *e, int kernel)
     return 0;
 }
 
+int sbk_get_launch(sbk_engine *e, int precision, int *tpt_out, int *groups_out, int *slices_out,
+                   int *local_out, int *kernel_out)
+{
+    int rc = prepare(e);
+    if (rc) return rc;
+    if (precision < SBK_FP64_EXACT || precision > SBK_FP64_RSQ)
+        return fail(SBK_EINVAL, "unknown precision %d", precision);
+    ItemList *L = nullptr;
+    if ((rc = get_list(e, precision, false, &L))) return rc;
+    const bool exact = precision == SBK_FP64_EXACT;
+    if (tpt_out) *tpt_out = exact ? 1 : targets_per_thread(e, precision);
+    if (groups_out) *groups_out = exact ? 1 : e->groups;
+    if (slices_out) *slices_out = L->max_slices;
+    if (local_out) *local_out = local_coords(e, precision);
+    if (kernel_out)
+        *kernel_out = coulomb_kernel(e, precision) == sbk::KK_STREAM ? SBK_KERNEL_STREAM : SBK_KERNEL_TILED;
+    return 0;
+}
+
 // Mean time of `reps` Coulomb launches on the current positions; with_sum
 // adds the pass that reads every slice partial back (what each stage kernel
 // pays), so launch shapes with many slices are charged for their partials.

@@ -384,6 +384,62 @@ def test_local_coords_close_pairs_take_the_fp64_separation(rng, precision, unit,
             assert per_ion.max() < 1e-6, (local, per_ion.max())
 
 
+def _sizes(kind):
+    if kind == "small ensemble":      # 512 clouds of 64-512 ions: 4e7 pairs
+        r = np.random.default_rng(11)
+        return np.round(np.exp(r.uniform(np.log(64), np.log(512), 512))).astype(int).tolist()
+    if kind == "ensemble":            # the 256-cloud set-up, 64-4,096 ions
+        return [c.n for c in synth.workload(4).clouds]
+    return [int(kind)]
+
+
+@pytest.mark.parametrize("kind,precision,want", [
+    ("500", "mixed", (1, 4, 0, 1)),
+    ("6000", "mixed", (1, 4, 0, 1)),
+    ("10000", "mixed", (2, 4, 1, 1)),
+    ("20000", "mixed", (2, 4, 1, 1)),
+    ("40000", "mixed", (4, 4, 1, 1)),
+    ("small ensemble", "mixed", (1, 4, 0, 1)),
+    ("ensemble", "mixed", (2, 4, 1, 1)),
+    ("10000", "fp64", (1, 4, 0, 0)),
+    ("40000", "fp64", (4, 4, 0, 0)),
+    ("ensemble", "fp64", (2, 4, 0, 0)),
+])
+def test_automatic_launch_shape(kind, precision, want):
+    """The automatic launch shape as sbk_set_launch documents it: 4 targets
+    per thread from a pair-weighted mean cloud size of 32,768 ions, else 2
+    from 2.5e8 pairs or with local coordinates, else 1; local coordinates
+    (mixed, spatial order) from 8e7 pairs and a pair-weighted mean size of
+    2,048 ions; the streamed kernel for the mixed precisions with 4 source
+    groups.  The resolved slice count is that of the work list."""
+    sizes = _sizes(kind)
+    print(f"automatic launch: {kind} ({len(sizes)} clouds, {sum(sizes)} ions), {precision}", flush=True)
+    f = synth.penning_fields()
+    with S.Engine(precision=precision) as e:
+        for i, n in enumerate(sizes):
+            c = synth.gaussian_cloud(n, 1e-3, 300.0, seed=100 + i)
+            e.add_cloud(c.pos, c.vel, c.mass, c.charge, f)
+        tpt, groups, slices, local, kernel = e.launch(precision)
+        assert (tpt, groups, local, kernel) == want
+        assert slices >= 1
+        # an explicit shape is reported back as set (groups 1: the tiled kernel)
+        e.set_launch(1, 1, 3, 0)
+        t1, g1, s1, l1, k1 = e.launch(precision)
+        assert (t1, g1, l1, k1) == (1, 1, 0, 0)
+        assert s1 >= 3 if min(sizes) >= 3 * 1024 else s1 >= 1
+
+
+def test_automatic_launch_shape_headline():
+    """65,536 ions, mixed: 4 targets per thread, 4 source groups, 32 slices
+    (eight rounds of the resident work-groups), local coordinates, the
+    streamed kernel: the shape the shipped table also records."""
+    c = synth.workload(3).clouds[0]
+    with S.Engine(precision="mixed") as e:
+        e.add_cloud(c.pos, c.vel, c.mass, c.charge, c.fields)
+        assert e.launch("mixed") == (4, 4, 32, 1, 1)
+        assert e.launch("fp64_exact")[:3] == (1, 1, 1)
+
+
 @pytest.mark.parametrize("tpt", [1, 2, 4, 8])
 @pytest.mark.parametrize("groups", [1, 4])
 @pytest.mark.parametrize("slices", [0, 1, 3, 7])
