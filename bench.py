@@ -422,7 +422,7 @@ def other_workload(S, w, args, precision):
                               "frac": tf / peak}
         e.close()
         log(f"{p}: {res[p]}")
-        if res["pairs_per_attempt"] < 1e10:
+        if res["pairs_per_attempt"] < 1e10 and args.replay:
             # the graph replay against persistent attempts, about 1 s of attempts
             k = max(args.steps, min(400, int(2e11 / res["pairs_per_attempt"])))
             res[p]["replay"] = replay(S, w, p, k, args.warmup, tune)
@@ -459,6 +459,9 @@ def main():
     ap.add_argument("--quick", action="store_true",
                     help="skip the trajectory-deviation and invariant-drift phases "
                          "(profiler runs of the timed region)")
+    ap.add_argument("--no-replay", dest="replay", action="store_false",
+                    help="skip the graph-replay against persistent-attempts comparison of workloads "
+                         "1, 2 and 4 (its cooperative launch ends rocprofv3 traces with a crash at exit)")
     ap.add_argument("--to-end", action="store_true",
                     help="also run the workload's whole simulated interval (mixed and fp64; "
                          "for the 65,536-ion workload, mixed only)")
