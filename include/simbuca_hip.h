@@ -193,7 +193,8 @@ typedef struct sbk_config {
                                 sub-tiles form separations from FP32 offsets
                                 (faster, same accuracy); results are returned
                                 in the caller's order either way */
-    int    reorder_interval; /* attempts between re-sorts of that order as the
+    int    reorder_interval; /* attempt launches (one attempt of every active
+                                cloud) between re-sorts of that order as the
                                 ions move (0: sort once, at the first run) */
     double beta;        /* PI control of the step controller: 0 (the reference's
                            value, default) or e.g. 0.04..0.08, the reference's

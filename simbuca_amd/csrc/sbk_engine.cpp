@@ -985,13 +985,22 @@ int reorder(sbk_engine *e)
     return upload_params(e);   // the slot of each cloud's ion 0 moved
 }
 
+// Attempts run so far as the re-sort interval counts them: one per launch of
+// the attempt sequence, which is the most attempts any cloud has made (a sum
+// over the clouds would re-sort an ensemble of C clouds C times as often).
+long long attempts_run(const sbk_engine *e)
+{
+    long long att = 0;
+    for (size_t i = 0; i < e->clouds.size(); ++i) att = std::max(att, e->h_state[i].attempts);
+    return att;
+}
+
 // Re-sorts when reorder_interval attempts have run since the last sort; the
 // per-cloud states must be fresh (read_states) and the stream idle.
 int maybe_reorder(sbk_engine *e)
 {
     if (!e->spatial || e->cfg.reorder_interval <= 0) return 0;
-    long long att = 0;
-    for (size_t i = 0; i < e->clouds.size(); ++i) att += e->h_state[i].attempts;
+    const long long att = attempts_run(e);
     if (att - e->attempts_at_order < e->cfg.reorder_interval) return 0;
     e->attempts_at_order = att;
     return reorder(e);
@@ -1331,9 +1340,7 @@ int sbk_reorder(sbk_engine *e)
     if (rc) return rc;
     rc = read_states(e);
     if (rc) return rc;
-    long long att = 0;
-    for (size_t i = 0; i < e->clouds.size(); ++i) att += e->h_state[i].attempts;
-    e->attempts_at_order = att;
+    e->attempts_at_order = attempts_run(e);
     return reorder(e);
 }
 

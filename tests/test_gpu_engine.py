@@ -625,6 +625,22 @@ def test_reorder_during_a_run(rng, precision):
     assert np.allclose(v1, ov, rtol=0, atol=1e-5)
 
 
+def test_reorder_interval_counts_launches_not_cloud_attempts(rng):
+    """reorder_interval counts attempt launches (one attempt of every active
+    cloud), not attempts summed over the clouds: 8 clouds, fixed steps, 64
+    attempts each and an interval of 16 allow at most 64 / 16 = 4 re-sorts
+    (a sum over the clouds would re-sort at every check of the run)."""
+    sf, _ = _both_fields(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0)
+    with S.Engine(precision="mixed", adaptive=False, h0=1e-9, reorder_interval=16) as e:
+        for _ in range(8):
+            amu, m, ch, pos, vel = _rand_cloud(rng, 300)
+            e.add_cloud(pos, vel, m, ch, sf)
+        e.advance(63.5e-9)
+        e.synchronize()
+        assert all(e.status(i)["attempts"] == 64 for i in range(8))
+        assert 1 <= e.stats()["reorders"] <= 4
+
+
 def test_multi_cloud_equals_single_clouds(rng):
     sizes = [64, 1, 300, 0, 1029]
     clouds = [_rand_cloud(rng, n) for n in sizes]
