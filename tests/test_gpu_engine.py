@@ -641,6 +641,32 @@ def test_reorder_interval_counts_launches_not_cloud_attempts(rng):
         assert 1 <= e.stats()["reorders"] <= 4
 
 
+def test_ensemble_advance_equals_clouds_alone(rng):
+    """An ensemble advanced together against each cloud advanced alone, FP64
+    exact (whose sums do not depend on the work list), bit for bit.  The clouds
+    need different numbers of attempts (the dense 900-ion one the most), so
+    during the run the work list is planned again for the clouds still
+    active; a cloud dropped from it, or a stale partial sum, would show."""
+    specs = [(900, 1e-4), (40, 5e-4), (200, 5e-4), (500, 3e-4)]
+    clouds = [_rand_cloud(rng, n, sigma=sg) for n, sg in specs]
+    sf, _ = _both_fields(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0)
+    kw = dict(precision="fp64_exact", rtol=1e-9, atol=1e-9, h0=1e-9, time_mode=S.TIME_STAGE)
+    with S.Engine(**kw) as e:
+        for _, m, ch, pos, vel in clouds:
+            e.add_cloud(pos, vel, m, ch, sf)
+        e.advance(2e-7)
+        together = [(*e.state(i), e.status(i)["steps"], e.status(i)["attempts"]) for i in range(len(specs))]
+    att = [t[3] for t in together]
+    assert att[0] == max(att) and att[0] > 1.5 * min(att), att
+    for (_, m, ch, pos, vel), (p0, v0, steps, attempts) in zip(clouds, together):
+        with S.Engine(**kw) as e:
+            e.add_cloud(pos, vel, m, ch, sf)
+            e.advance(2e-7)
+            p1, v1 = e.state(0)
+            assert (e.status(0)["steps"], e.status(0)["attempts"]) == (steps, attempts)
+        assert np.array_equal(p0, p1) and np.array_equal(v0, v1)
+
+
 def test_multi_cloud_equals_single_clouds(rng):
     sizes = [64, 1, 300, 0, 1029]
     clouds = [_rand_cloud(rng, n) for n in sizes]
