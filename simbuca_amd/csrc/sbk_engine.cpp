@@ -1350,6 +1350,24 @@ int sbk_step(sbk_engine *e)
     return fail(SBK_ESTATE, "no accepted step after 100000 attempts");
 }
 
+// Attempts for the next batch of a run: as many as the active cloud that
+// needs the most still wants at its current step size (to its end time, or
+// its step budget), plus one for a rejection, within `cap`.  Launches after
+// every cloud has finished do nothing, yet each costs its 16 kernels.
+static int next_batch(const sbk_engine *e, int cap)
+{
+    double need = 0.0;
+    for (size_t i = 0; i < e->clouds.size(); ++i) {
+        const sbk::CloudState &s = e->h_state[i];
+        if (!s.active) continue;
+        double n = (std::isfinite(s.t_end) && s.h != 0.0) ? std::ceil((s.t_end - s.t) / std::fabs(s.h))
+                                                          : (double)cap;
+        if (s.budget > 0) n = std::min(n, (double)s.budget);
+        need = std::max(need, n);
+    }
+    return (int)std::max(1.0, std::min((double)cap, need + 1.0));
+}
+
 static int advance_impl(sbk_engine *e, double value, int set_end, long long max_attempts)
 {
     set_list_mask(e, {});   // every cloud starts the run
@@ -1358,7 +1376,7 @@ static int advance_impl(sbk_engine *e, double value, int set_end, long long max_
         return fail(SBK_EHIP, "begin kernel launch failed");
     int rc = 0;
     long long launched = 0;
-    int batch = 4;
+    int batch = 4, cap = 4;   // the cap doubles to 64 as the run goes on
     while (launched < max_attempts) {
         const int k = (int)std::min<long long>(batch, max_attempts - launched);
         rc = run_attempts(e, k);
@@ -1374,7 +1392,8 @@ static int advance_impl(sbk_engine *e, double value, int set_end, long long max_
         rc = maybe_reorder(e);
         if (rc) break;
         narrow_list(e);
-        batch = std::min(batch * 2, 64);
+        cap = std::min(cap * 2, 64);
+        batch = next_batch(e, cap);
     }
     // other callers (sbk_attempts, the evaluations) see the list of all clouds
     if (e->stream) (void)hipStreamSynchronize(e->stream);
