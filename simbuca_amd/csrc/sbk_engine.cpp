@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <thread>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -1008,20 +1009,49 @@ int reorder(sbk_engine *e)
                                hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     std::vector<int> src(n);
-    std::vector<double> aos;
-    for (auto &c : e->clouds) {
-        aos.resize(3 * (size_t)c.n);
-        for (int i = 0; i < c.n; ++i)
-            for (int k = 0; k < 3; ++k) aos[3 * (size_t)i + k] = soa[k * S + c.off + i];
-        const std::vector<int> ord = spatial_order(aos.data(), c.n);
-        std::vector<int> perm(c.n);
-        for (int i = 0; i < c.n; ++i) {
-            src[c.off + i] = c.off + ord[i];
-            perm[i] = caller_index(c, ord[i]);
+    // each cloud's order depends on its own positions only, and it writes its
+    // own range of src: the clouds of an ensemble are shared among host
+    // threads in contiguous runs of about equal bisection work (n log n)
+    auto sort_clouds = [&](size_t c0, size_t c1) {
+        std::vector<double> aos;
+        for (size_t ci = c0; ci < c1; ++ci) {
+            HostCloud &c = e->clouds[ci];
+            aos.resize(3 * (size_t)c.n);
+            for (int i = 0; i < c.n; ++i)
+                for (int k = 0; k < 3; ++k) aos[3 * (size_t)i + k] = soa[k * S + c.off + i];
+            const std::vector<int> ord = spatial_order(aos.data(), c.n);
+            std::vector<int> perm(c.n);
+            for (int i = 0; i < c.n; ++i) {
+                src[c.off + i] = c.off + ord[i];
+                perm[i] = caller_index(c, ord[i]);
+            }
+            apply_order(c.mass, ord, 1);
+            apply_order(c.charge, ord, 1);
+            c.perm.swap(perm);
         }
-        apply_order(c.mass, ord, 1);
-        apply_order(c.charge, ord, 1);
-        c.perm.swap(perm);
+    };
+    const size_t nc = e->clouds.size();
+    size_t nt = std::min<size_t>({(size_t)std::max(1u, std::thread::hardware_concurrency()),
+                                  (size_t)16, nc});
+    if (n < 16384) nt = 1;   // thread start-up would cost more than the sort
+    if (nt <= 1) {
+        sort_clouds(0, nc);
+    } else {
+        std::vector<double> work(nc + 1, 0.0);
+        for (size_t ci = 0; ci < nc; ++ci) {
+            const double m = (double)std::max(1, e->clouds[ci].n);
+            work[ci + 1] = work[ci] + m * std::log2(m + 1.0);
+        }
+        std::vector<std::thread> pool;
+        size_t c0 = 0;
+        for (size_t t = 1; t <= nt && c0 < nc; ++t) {
+            size_t c1 = c0;
+            while (c1 < nc && (t == nt || work[c1 + 1] * nt <= work[nc] * t)) ++c1;
+            if (c1 == c0) c1 = c0 + 1;
+            pool.emplace_back(sort_clouds, c0, c1);
+            c0 = c1;
+        }
+        for (auto &th : pool) th.join();
     }
     HIP_TRY(hipMemcpyAsync(e->d_src, src.data(), sizeof(int) * n, hipMemcpyHostToDevice, e->stream));
     double *base = e->d_f64;
