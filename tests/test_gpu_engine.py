@@ -667,6 +667,33 @@ def test_ensemble_advance_equals_clouds_alone(rng):
         assert np.array_equal(p0, p1) and np.array_equal(v0, v1)
 
 
+def test_reorder_of_an_ensemble_keeps_every_cloud(rng):
+    """A re-sort of an ensemble large enough that its clouds are bisected on
+    several host threads (24 clouds, about 20,000 ions), after the ions have
+    moved: every cloud's state, returned in the caller's order, is unchanged
+    bit for bit, and its forces (charges and masses follow their ions) agree
+    with those before the re-sort to the mixed precision's rounding."""
+    sizes = [int(x) for x in rng.integers(300, 1400, 24)]
+    assert sum(sizes) >= 16384
+    clouds = [_rand_cloud(rng, n, sigma=4e-4, species=(133.0, 85.0), charges=(1, 2)) for n in sizes]
+    sf, _ = _both_fields(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0)
+    with S.Engine(precision="mixed", reorder_interval=0) as e:
+        for _, m, ch, pos, vel in clouds:
+            e.add_cloud(pos, vel, m, ch, sf)
+        e.advance(1e-6)             # the ions move about a third of the cloud's width
+        before = [(*e.state(i), e.eval_force(i), e.eval_coulomb(i, "mixed"))
+                  for i in range(len(sizes))]
+        e.reorder()
+        assert e.stats()["reorders"] >= 1
+        changed = 0
+        for i, (p0, v0, f0, c0) in enumerate(before):
+            p1, v1 = e.state(i)
+            assert np.array_equal(p0, p1) and np.array_equal(v0, v1), i
+            assert _rows_error(e.eval_force(i), f0) < 1e-6, i
+            changed += not np.array_equal(e.eval_coulomb(i, "mixed"), c0)
+        assert changed > 0          # the new order sums in another order
+
+
 def test_multi_cloud_equals_single_clouds(rng):
     sizes = [64, 1, 300, 0, 1029]
     clouds = [_rand_cloud(rng, n) for n in sizes]
