@@ -465,6 +465,11 @@ int get_list(sbk_engine *e, int precision, bool only_coulomb, ItemList **out)
     // mid-size clouds may take half-size blocks (sbk_plan.h)
     const int split_blk = (kern == sbk::KK_STREAM && tpt == 4) ? blk / 2 : 0;
     sbk::Plan plan = sbk::plan_coulomb(pc, blk, sbk::kSubTile * q, sreq, capacity, exact, split_blk);
+    // a list narrowed to the clouds still active keeps the requested slice
+    // count, chosen for all clouds; when its items would not fill one round
+    // of resident work-groups, the automatic split serves it better
+    if (mask_id != 0 && !exact && sreq > 0 && (long long)plan.items.size() < capacity)
+        plan = sbk::plan_coulomb(pc, blk, sbk::kSubTile * q, 0, capacity, exact, split_blk);
     const std::string bad = sbk::check_plan(pc, plan);
     if (!bad.empty()) return fail(SBK_ESTATE, "Coulomb work list is inconsistent: %s", bad.c_str());
     ItemList L;
