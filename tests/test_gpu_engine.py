@@ -647,18 +647,21 @@ def test_ensemble_advance_equals_clouds_alone(rng):
     need different numbers of attempts (the dense 900-ion one the most), so
     during the run the work list is planned again for the clouds still
     active; a cloud dropped from it, or a stale partial sum, would show."""
-    specs = [(900, 1e-4), (40, 5e-4), (200, 5e-4), (500, 3e-4)]
-    clouds = [_rand_cloud(rng, n, sigma=sg) for n, sg in specs]
-    sf, _ = _both_fields(trap=0, coulomb=1, V0=10.0, d0=11e-3, Bz=7.0)
+    # (ions, sigma, Coulomb term): a single ion and a cloud without Coulomb
+    # term are in the work list's mask as any other cloud
+    specs = [(900, 1e-4, 1), (40, 5e-4, 1), (200, 5e-4, 1), (500, 3e-4, 1), (1, 5e-4, 1),
+             (300, 3e-4, 0)]
+    clouds = [_rand_cloud(rng, n, sigma=sg) for n, sg, _ in specs]
+    fields = [_both_fields(trap=0, coulomb=cb, V0=10.0, d0=11e-3, Bz=7.0)[0] for _, _, cb in specs]
     kw = dict(precision="fp64_exact", rtol=1e-9, atol=1e-9, h0=1e-9, time_mode=S.TIME_STAGE)
     with S.Engine(**kw) as e:
-        for _, m, ch, pos, vel in clouds:
+        for (_, m, ch, pos, vel), sf in zip(clouds, fields):
             e.add_cloud(pos, vel, m, ch, sf)
         e.advance(2e-7)
         together = [(*e.state(i), e.status(i)["steps"], e.status(i)["attempts"]) for i in range(len(specs))]
     att = [t[3] for t in together]
     assert att[0] == max(att) and att[0] > 1.5 * min(att), att
-    for (_, m, ch, pos, vel), (p0, v0, steps, attempts) in zip(clouds, together):
+    for (_, m, ch, pos, vel), sf, (p0, v0, steps, attempts) in zip(clouds, fields, together):
         with S.Engine(**kw) as e:
             e.add_cloud(pos, vel, m, ch, sf)
             e.advance(2e-7)
